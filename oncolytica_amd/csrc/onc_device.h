@@ -1,0 +1,274 @@
+// onc_device.h — device-side definitions shared by the fixed gfx950 kernels
+// (onc_kernels.hip) and by rule modules that the Python translator emits and
+// hiprtc compiles. Self-contained: no standard headers, so hiprtc can include
+// it as a named header.
+//
+// Floating-point expressions here are evaluated in the exact order of the CPU
+// oracle; the library and every rule module are compiled with
+// -ffp-contract=off and IEEE division / square root, so binary32 results are
+// bit-identical to the oracle's.
+#pragma once
+
+typedef unsigned int onc_u32;
+typedef int onc_i32;
+typedef unsigned long long onc_u64;
+
+#define ONC_DEV_MAX_FIELDS 4
+#define ONC_DEV_MAX_COLS 48
+#define ONC_DEV_MAX_PARAMS 32
+#define ONC_DEV_MAX_PASSES 4
+
+#define ONC_FLAG_ALIVE 1u
+#define ONC_FLAG_DIV 2u
+
+// Mirrors onc_field_desc of include/onco.h (layout checked by static_assert).
+struct onc_dfield {
+    float rate, decay, init;
+    int bc;
+    float bc_value, src_radius, uptake, lo, hi;
+};
+
+// Dynamic per-sim counters, updated on the device only (graph replays need
+// no host round trip).
+struct onc_dstate {
+    onc_u32 n;        // live cells
+    onc_u32 next_id;  // id of the next daughter
+    onc_u32 step;     // completed steps (Philox counter word 1)
+    onc_u32 parity;   // which state / field buffer is current
+    onc_u32 nsurv, ndiv;           // written by the compaction scan
+    onc_u32 pend_n, pend_next_id;  // written by the daughter append
+    onc_u32 pending;               // a step is in flight, commit before use
+    onc_u32 pad0;
+    onc_u64 cell_updates;          // sum over steps of live cells at step start
+    onc_u32 pad[4];
+};
+
+// One entry of the packed segment table: a sim's geometry, model constants
+// and device buffers.
+struct onc_dsim {
+    int dims[3];
+    float dx;
+    int key_format;
+    int key_bits;
+    onc_u32 cap;
+    onc_u32 seed_lo, seed_hi;
+    float cell_radius, R, k_rep, k_adh, mobility, max_disp;
+    int n_fields, n_cols;
+    onc_dfield fields[ONC_DEV_MAX_FIELDS];
+    float params[ONC_DEV_MAX_PARAMS];
+    onc_dstate* state;
+    onc_u32* st[2][ONC_DEV_MAX_COLS];  // cell state columns, two buffers
+    onc_u32* dau[ONC_DEV_MAX_COLS];    // rule output: daughter of cell i at i
+    onc_u32* dau2[ONC_DEV_MAX_COLS];   // daughters compacted in parent order
+    float* fld[2][ONC_DEV_MAX_FIELDS]; // fields, two buffers, x + X (y + Y z)
+    onc_u32* keys[2];
+    onc_u32* vals[2];
+    onc_u32* flags;
+    onc_u32* bin_start;  // indexed by Morton key; valid iff start < n and
+    onc_u32* bin_end;    //   keys_sorted[start] == key
+};
+
+// Arguments of every batched kernel.
+struct onc_kargs {
+    const onc_dsim* sims;
+    onc_u32 nsims;
+    onc_u32 passes;       // radix passes this step (max over sims)
+    onc_u32 tile;         // cells per sort / compaction tile
+    onc_u32 max_tiles;    // tiles the status arrays hold
+    onc_u32* tile_prefix; // [nsims + 1] tile prefix of live counts
+    onc_u32* cell_prefix; // [nsims + 1] prefix of live counts
+    onc_u32* hist;        // [nsims][4][256] digit histograms
+    onc_u32* counters;    // [16] dynamic tile counters
+    onc_u32* sort_status; // [4][max_tiles][256] look-back words
+    onc_u64* comp_status; // [max_tiles] look-back words of the compaction scan
+    onc_u32* jobs_prefix; // [njobs + 1] stencil blocks per (sim, field)
+    onc_u32* jobs;        // [njobs] sim << 8 | field
+    onc_u32 njobs;
+    int sbx, sby, sbz;    // stencil block shape
+    onc_u32* err;         // sticky error word (bounded waits that expired)
+};
+
+// ---- small helpers ---------------------------------------------------------
+__device__ __forceinline__ float onc_clampf(float x, float lo, float hi) {
+    return x < lo ? lo : (x > hi ? hi : x);
+}
+
+// Voxel coordinate along one axis: floor of the IEEE quotient, clamped to
+// [0, d-1] (floor and trunc agree on positions >= 0).
+__device__ __forceinline__ int onc_vox1(float p, float dx, int d) {
+    float q = p / dx;
+    float f = __builtin_floorf(q);
+    return f < 0.0f ? 0 : (f >= (float)d ? d - 1 : (int)f);
+}
+
+__device__ __forceinline__ onc_u32 onc_spread9(onc_u32 v) {
+    v &= 0x1FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// 3-D Morton key, 10 bits per axis (framework/src/oncolytica/core/runtime/
+// _geometry.py:14-42 closed form); fmt 32 adds x bit 10 -> 30, y bit 10 -> 31.
+__device__ __forceinline__ onc_u32 onc_morton(onc_u32 x, onc_u32 y, onc_u32 z, int fmt) {
+    onc_u32 xl = x & 0x3FFu, yl = y & 0x3FFu, zl = z & 0x3FFu;
+    onc_u32 j = onc_spread9(xl >> 1) | (onc_spread9(yl >> 1) << 1) | (onc_spread9(zl >> 1) << 2);
+    onc_u32 k = (j << 3) | (xl & 1u) | ((yl & 1u) << 1) | ((zl & 1u) << 2);
+    if (fmt == 32) k |= (((x >> 10) & 1u) << 30) | (((y >> 10) & 1u) << 31);
+    return k;
+}
+
+__device__ __forceinline__ onc_u32 onc_cell_key(const onc_dsim& S, float px, float py, float pz) {
+    return onc_morton((onc_u32)onc_vox1(px, S.dx, S.dims[0]), (onc_u32)onc_vox1(py, S.dx, S.dims[1]),
+                      (onc_u32)onc_vox1(pz, S.dx, S.dims[2]), S.key_format);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ void onc_philox(onc_u32 c0, onc_u32 c1, onc_u32 c2, onc_u32 c3, onc_u32 k0,
+                                           onc_u32 k1, onc_u32 out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        onc_u32 hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        onc_u32 hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        onc_u32 n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+// Per-cell counter-based stream: call c reads word c & 3 of
+// Philox(ctr = {id, step, c >> 2, 0}, key = seed).
+struct onc_rng {
+    onc_u32 id, step, c, k0, k1;
+    onc_u32 buf[4];
+};
+
+__device__ __forceinline__ void onc_rng_init(onc_rng& r, const onc_dsim& S, onc_u32 id, onc_u32 step) {
+    r.id = id;
+    r.step = step;
+    r.c = 0;
+    r.k0 = S.seed_lo;
+    r.k1 = S.seed_hi;
+}
+
+__device__ __forceinline__ float onc_rand(onc_rng& r) {
+    if ((r.c & 3u) == 0) onc_philox(r.id, r.step, r.c >> 2, 0u, r.k0, r.k1, r.buf);
+    onc_u32 w = r.buf[r.c & 3u];
+    r.c++;
+    return (float)(w >> 8) * 5.9604644775390625e-08f;
+}
+
+// sin, cos on [0, pi/2] from the Taylor series (Horner in a^2), the same
+// operation sequence as the oracle.
+__device__ __forceinline__ void onc_sincos_q(float a, float* so, float* co) {
+    float a2 = a * a;
+    float s = (float)1.6059043836821613e-10;
+    s = s * a2 + (float)-2.5052108385441720e-08;
+    s = s * a2 + (float)2.7557319223985893e-06;
+    s = s * a2 + (float)-1.9841269841269841e-04;
+    s = s * a2 + (float)8.3333333333333333e-03;
+    s = s * a2 + (float)-1.6666666666666667e-01;
+    s = s * a2 + 1.0f;
+    s = s * a;
+    float c = (float)-1.1470745597729725e-11;
+    c = c * a2 + (float)2.0876756987868099e-09;
+    c = c * a2 + (float)-2.7557319223985891e-07;
+    c = c * a2 + (float)2.4801587301587302e-05;
+    c = c * a2 + (float)-1.3888888888888889e-03;
+    c = c * a2 + (float)4.1666666666666667e-02;
+    c = c * a2 + -0.5f;
+    c = c * a2 + 1.0f;
+    *so = s;
+    *co = c;
+}
+
+// Unit vector: z = 2 u1 - 1, phi = 2 pi u2 with exact quadrant reduction
+// (gpu/compiler/template.wgsl:84-98 z/phi form).
+__device__ __forceinline__ void onc_rand_dir(onc_rng& r, float* x, float* y, float* z) {
+    float u1 = onc_rand(r);
+    float u2 = onc_rand(r);
+    float zz = u1 * 2.0f - 1.0f;
+    float rr = 1.0f - zz * zz;
+    if (rr < 0.0f) rr = 0.0f;
+    float rad = __builtin_sqrtf(rr);
+    float t = u2 * 4.0f;
+    int q = (int)t;
+    float f = t - (float)q;
+    float a = f * (float)1.5707963267948966;
+    float s, c;
+    onc_sincos_q(a, &s, &c);
+    float cx, sy;
+    switch (q & 3) {
+        case 0: cx = c; sy = s; break;
+        case 1: cx = -s; sy = c; break;
+        case 2: cx = -c; sy = -s; break;
+        default: cx = s; sy = -c; break;
+    }
+    *x = rad * cx;
+    *y = rad * sy;
+    *z = zz;
+}
+
+// Largest s with prefix[s] <= g (prefix is non-decreasing, prefix[0] = 0).
+__device__ __forceinline__ int onc_find_seg(const onc_u32* prefix, onc_u32 nsegs, onc_u32 g) {
+    int lo = 0, hi = (int)nsegs - 1;
+    while (lo < hi) {
+        int m = (lo + hi + 1) >> 1;
+        if (prefix[m] <= g) lo = m; else hi = m - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ onc_u32 onc_field_index(const onc_dsim& S, float px, float py, float pz) {
+    onc_u32 X = (onc_u32)S.dims[0], Y = (onc_u32)S.dims[1];
+    return (onc_u32)onc_vox1(px, S.dx, S.dims[0]) +
+           X * ((onc_u32)onc_vox1(py, S.dx, S.dims[1]) + Y * (onc_u32)onc_vox1(pz, S.dx, S.dims[2]));
+}
+
+// Iterate the cells of the 27 Moore bins around (px, py, pz) in
+// template.wgsl:11-21 order (x fastest), skipping out-of-range bins and the
+// cell `self`. `body(j)` receives the sorted index of each neighbour.
+template <class F>
+__device__ __forceinline__ void onc_for_neighbors(const onc_dsim& S, const onc_u32* keys, onc_u32 n, float px,
+                                                  float py, float pz, onc_u32 self, F body) {
+    int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]), vz = onc_vox1(pz, S.dx, S.dims[2]);
+    for (int oz = -1; oz <= 1; oz++) {
+        int nz = vz + oz;
+        if (nz < 0 || nz >= S.dims[2]) continue;
+        for (int oy = -1; oy <= 1; oy++) {
+            int ny = vy + oy;
+            if (ny < 0 || ny >= S.dims[1]) continue;
+            for (int ox = -1; ox <= 1; ox++) {
+                int nx = vx + ox;
+                if (nx < 0 || nx >= S.dims[0]) continue;
+                onc_u32 key = onc_morton((onc_u32)nx, (onc_u32)ny, (onc_u32)nz, S.key_format);
+                onc_u32 s = S.bin_start[key];
+                if (s >= n || keys[s] != key) continue;
+                onc_u32 e = S.bin_end[key];
+                for (onc_u32 j = s; j < e; j++) {
+                    if (j == self) continue;
+                    body(j);
+                }
+            }
+        }
+    }
+}
+
+// Arguments of a rule kernel (built-in or hiprtc module): one thread per live
+// cell of every sim in the segment table.
+struct onc_rule_args {
+    const onc_dsim* sims;
+    onc_u32 nsims;
+    onc_u32 passes;
+    const onc_u32* cell_prefix;
+};

@@ -1,0 +1,56 @@
+// onc_internal.h — host-side declarations shared by the kernel launchers
+// (onc_kernels.hip) and the C ABI runtime (onc_runtime.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "onc_device.h"
+
+#define ONC_CTR_SORT0 0   // counters[0..3]: sort passes
+#define ONC_CTR_COMPACT 4 // counters[4]: compaction scan
+#define ONC_CTR_SCAN 5    // counters[5]: standalone exclusive scan
+#define ONC_NCOUNTERS 16
+
+#define ONC_BLOCK 256
+
+// Phase ids for per-phase timing (onc_batch_phase_times).
+enum onc_phase {
+    ONC_PH_PLAN = 0,
+    ONC_PH_MORTON = 1,
+    ONC_PH_SORT = 2,
+    ONC_PH_PERMUTE_BINS = 3,
+    ONC_PH_FORCES = 4,
+    ONC_PH_STENCIL = 5,
+    ONC_PH_RULE = 6,
+    ONC_PH_COMPACT = 7,
+    ONC_PH_DAUGHTERS = 8,
+    ONC_PH_OTHER = 9,
+};
+
+// Plan / commit modes of k_plan.
+#define ONC_PLAN_STEP 0
+#define ONC_PLAN_COMMIT 1
+
+// Launchers (all asynchronous on `s`). `grid_cells` / `grid_tiles` are the
+// persistent grid sizes chosen by the runtime.
+hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s);
+hipError_t onc_launch_morton(const onc_kargs& a, int items, int grid, hipStream_t s);
+hipError_t onc_launch_hist_only(const onc_kargs& a, int items, int grid, hipStream_t s);
+hipError_t onc_launch_sort_pass(const onc_kargs& a, int items, int pass, int grid, hipStream_t s);
+hipError_t onc_launch_permute_bins(const onc_kargs& a, int grid, hipStream_t s);
+hipError_t onc_launch_forces(const onc_kargs& a, int grid, hipStream_t s);
+hipError_t onc_launch_stencil(const onc_kargs& a, onc_u32 nblocks, hipStream_t s);
+hipError_t onc_launch_rule_builtin(const onc_rule_args& r, int grid, hipStream_t s);
+hipError_t onc_launch_compact(const onc_kargs& a, int items, int grid, hipStream_t s);
+hipError_t onc_launch_daughters(const onc_kargs& a, int grid, hipStream_t s);
+
+// pure kernels
+hipError_t onc_launch_morton_keys(const float* px, const float* py, const float* pz, onc_u32 n, int X,
+                                  int Y, int Z, float dx, int fmt, onc_u32* keys, hipStream_t s);
+hipError_t onc_launch_bins(const onc_u32* keys, onc_u32 n, onc_u32* start, onc_u32* end, hipStream_t s);
+hipError_t onc_launch_philox(const onc_u32* ctr, onc_u32 n, onc_u32 k0, onc_u32 k1, onc_u32* out,
+                             hipStream_t s);
+hipError_t onc_launch_exscan(const onc_u32* in, onc_u32* out, onc_u32 n, onc_u32* total,
+                             onc_u32* status, onc_u32* counter, onc_u32* err, int grid, hipStream_t s);
+hipError_t onc_launch_fill_u32(onc_u32* p, onc_u32 v, size_t n, hipStream_t s);
+hipError_t onc_launch_field_init(float* u, int X, int Y, int Z, const onc_dfield& f, hipStream_t s);
+hipError_t onc_launch_copy_u32(const onc_u32* src, onc_u32* dst, size_t n, hipStream_t s);

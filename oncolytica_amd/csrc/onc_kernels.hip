@@ -1,0 +1,938 @@
+// onc_kernels.hip — fixed gfx950 kernels of the per-timestep hot path.
+//
+// Every kernel serves a whole batch: it walks a packed segment table of sims
+// (onc_dsim) and finds its sim by binary search over device-side prefixes of
+// the live counts, so one launch per phase covers sims of uneven sizes and a
+// captured hipGraph replays without host round trips. Kernels over cells are
+// persistent: a grid sized to the machine loops over tiles.
+//
+// Replaces the WGSL template kernels of the reference
+// (framework/src/oncolytica/gpu/compiler/template.wgsl:152-343): the atomic
+// counting sort into a Morton-indexed voxel table (Kernel_ClearVoxels,
+// Kernel_CountAndOffset, Kernel_Scan_L0/L1/L2, Kernel_AddBack_L1/L0,
+// Kernel_Scatter, Kernel_UpdateState) becomes a stable onesweep LSD radix
+// sort, bins built from the sorted keys, and flag-scan compaction.
+#include "onc_internal.h"
+
+#define ONC_FLAG_A 0x40000000u
+#define ONC_FLAG_P 0x80000000u
+#define ONC_CNT_MASK 0x3FFFFFFFu
+#define ONC_SPIN_LIMIT (1u << 24)
+
+static_assert(sizeof(onc_dfield) == 36, "onc_dfield layout");
+
+// ---------------------------------------------------------------------------
+// wave64 / block helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ onc_u32 ld_relaxed(const onc_u32* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(onc_u32* p, onc_u32 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ onc_u64 ld_relaxed64(const onc_u64* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed64(onc_u64* p, onc_u64 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive scan of one value per thread over a 256-thread block (4 wave64).
+// All threads must call it; *total receives the block sum.
+__device__ __forceinline__ onc_u32 block_exscan256(onc_u32 v, onc_u32* s_tmp, onc_u32* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    onc_u32 x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        onc_u32 y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[w] = x;
+    __syncthreads();
+    onc_u32 before = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        onc_u32 t = s_tmp[i];
+        if (i < w) before += t;
+        tot += t;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return before + x - v;
+}
+
+// ---------------------------------------------------------------------------
+// plan / commit (one workgroup): commits the previous step's counters, plans
+// this step's tiles from the live counts, clears histograms and tile counters.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
+    __shared__ onc_u32 s_tmp[8];
+    __shared__ onc_u32 s_carry_t, s_carry_c;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_carry_t = 0;
+        s_carry_c = 0;
+    }
+    __syncthreads();
+    for (onc_u32 b0 = 0; b0 < a.nsims; b0 += 256) {
+        onc_u32 s = b0 + tid;
+        onc_u32 nt = 0, nc = 0;
+        if (s < a.nsims) {
+            onc_dstate* st = a.sims[s].state;
+            if (st->pending) {
+                st->parity ^= 1u;
+                st->step += 1u;
+                st->n = st->pend_n;
+                st->next_id = st->pend_next_id;
+                st->pending = 0u;
+            }
+            if (mode == ONC_PLAN_STEP) {
+                st->pend_n = 0u;
+                st->pend_next_id = st->next_id;
+                st->nsurv = 0u;
+                st->ndiv = 0u;
+                st->pending = 1u;
+                st->cell_updates += st->n;
+                nc = st->n;
+                nt = (nc + a.tile - 1u) / a.tile;
+            }
+        }
+        if (mode == ONC_PLAN_STEP) {
+            onc_u32 tt, tc;
+            onc_u32 et = block_exscan256(nt, s_tmp, &tt);
+            onc_u32 ec = block_exscan256(nc, s_tmp, &tc);
+            if (s < a.nsims) {
+                a.tile_prefix[s] = s_carry_t + et;
+                a.cell_prefix[s] = s_carry_c + ec;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                s_carry_t += tt;
+                s_carry_c += tc;
+            }
+            __syncthreads();
+        }
+    }
+    if (mode == ONC_PLAN_STEP) {
+        if (tid == 0) {
+            a.tile_prefix[a.nsims] = s_carry_t;
+            a.cell_prefix[a.nsims] = s_carry_c;
+        }
+        for (onc_u32 i = tid; i < a.nsims * 4u * 256u; i += 256) a.hist[i] = 0u;
+        for (int i = tid; i < ONC_NCOUNTERS; i += 256) a.counters[i] = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Morton keys + digit histograms of every radix pass (one read of positions);
+// also clears this tile's look-back words for the sort and compaction scans.
+// ---------------------------------------------------------------------------
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
+    __shared__ onc_u32 s_hist[ONC_DEV_MAX_PASSES * 256];
+    const int tid = threadIdx.x;
+    const onc_u32 total = a.tile_prefix[a.nsims];
+    const onc_u32 P = a.passes;
+    for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
+        const int s = onc_find_seg(a.tile_prefix, a.nsims, g);
+        const onc_dsim& S = a.sims[s];
+        const onc_u32 n = S.state->n;
+        const onc_u32 par = S.state->parity;
+        const onc_u32 base = (g - a.tile_prefix[s]) * (onc_u32)(256 * ITEMS);
+        for (onc_u32 i = tid; i < P * 256u; i += 256) s_hist[i] = 0u;
+        __syncthreads();
+        const float* px = (const float*)S.st[par][0];
+        const float* py = (const float*)S.st[par][1];
+        const float* pz = (const float*)S.st[par][2];
+        onc_u32* ko = S.keys[0];
+        onc_u32* vo = S.vals[0];
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            onc_u32 i = base + (onc_u32)(k * 256 + tid);
+            if (i < n) {
+                onc_u32 key = onc_cell_key(S, px[i], py[i], pz[i]);
+                ko[i] = key;
+                vo[i] = i;
+                for (onc_u32 p = 0; p < P; p++) atomicAdd(&s_hist[p * 256u + ((key >> (8u * p)) & 255u)], 1u);
+            }
+        }
+        __syncthreads();
+        for (onc_u32 p = 0; p < P; p++) {
+            onc_u32 v = s_hist[p * 256u + tid];
+            if (v) atomicAdd(&a.hist[((size_t)s * 4u + p) * 256u + tid], v);
+            a.sort_status[((size_t)p * a.max_tiles + g) * 256u + tid] = 0u;
+        }
+        if (tid == 0) a.comp_status[g] = 0ull;
+        __syncthreads();
+    }
+}
+
+// Histograms only (keys already in keys[0]); used by onc_sort_pairs.
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_hist_only(onc_kargs a) {
+    __shared__ onc_u32 s_hist[ONC_DEV_MAX_PASSES * 256];
+    const int tid = threadIdx.x;
+    const onc_u32 total = a.tile_prefix[a.nsims];
+    const onc_u32 P = a.passes;
+    for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
+        const int s = onc_find_seg(a.tile_prefix, a.nsims, g);
+        const onc_dsim& S = a.sims[s];
+        const onc_u32 n = S.state->n;
+        const onc_u32 base = (g - a.tile_prefix[s]) * (onc_u32)(256 * ITEMS);
+        for (onc_u32 i = tid; i < P * 256u; i += 256) s_hist[i] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            onc_u32 i = base + (onc_u32)(k * 256 + tid);
+            if (i < n) {
+                onc_u32 key = S.keys[0][i];
+                for (onc_u32 p = 0; p < P; p++) atomicAdd(&s_hist[p * 256u + ((key >> (8u * p)) & 255u)], 1u);
+            }
+        }
+        __syncthreads();
+        for (onc_u32 p = 0; p < P; p++) {
+            onc_u32 v = s_hist[p * 256u + tid];
+            if (v) atomicAdd(&a.hist[((size_t)s * 4u + p) * 256u + tid], v);
+            a.sort_status[((size_t)p * a.max_tiles + g) * 256u + tid] = 0u;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One stable LSD radix pass (8-bit digit), onesweep style: wave64 ballot
+// multi-split ranks each digit inside the tile in input order, a decoupled
+// look-back over the sim's earlier tiles gives the tile's global offset per
+// digit, and an LDS reorder makes the scatter write runs of equal digits.
+// ---------------------------------------------------------------------------
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
+    constexpr int TILE = 256 * ITEMS;
+    __shared__ onc_u32 s_keys[TILE];
+    __shared__ onc_u32 s_vals[TILE];
+    __shared__ onc_u32 s_wc[4 * 256];
+    __shared__ onc_u32 s_texcl[256];
+    __shared__ onc_u32 s_gbase[256];
+    __shared__ onc_u32 s_tmp[8];
+    __shared__ onc_u32 s_tile;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const onc_u32 total = a.tile_prefix[a.nsims];
+    const onc_u32 shift = 8u * (onc_u32)pass;
+    const onc_u64 lt_mask = (1ull << lane) - 1ull;
+    onc_u32* status = a.sort_status + (size_t)pass * a.max_tiles * 256u;
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(&a.counters[ONC_CTR_SORT0 + pass], 1u);
+        for (int i = tid; i < 4 * 256; i += 256) s_wc[i] = 0u;
+        __syncthreads();
+        const onc_u32 g = s_tile;
+        if (g >= total) break;
+        const int s = onc_find_seg(a.tile_prefix, a.nsims, g);
+        const onc_dsim& S = a.sims[s];
+        const onc_u32 t = g - a.tile_prefix[s];
+        const onc_u32 n = S.state->n;
+        const onc_u32 base = t * (onc_u32)TILE;
+        const onc_u32 cnt = min((onc_u32)TILE, n - base);
+        const onc_u32* kin = S.keys[pass & 1];
+        const onc_u32* vin = S.vals[pass & 1];
+        onc_u32* kout = S.keys[(pass + 1) & 1];
+        onc_u32* vout = S.vals[(pass + 1) & 1];
+
+        onc_u32 key[ITEMS], val[ITEMS], rnk[ITEMS];
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            onc_u32 idx = (onc_u32)(w * 64 * ITEMS + k * 64 + lane);
+            key[k] = 0u;
+            val[k] = 0u;
+            if (idx < cnt) {
+                key[k] = kin[base + idx];
+                val[k] = vin[base + idx];
+            }
+        }
+        // rank inside the wave, items in (k, lane) order = input order
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            onc_u32 idx = (onc_u32)(w * 64 * ITEMS + k * 64 + lane);
+            bool valid = idx < cnt;
+            onc_u32 d = (key[k] >> shift) & 255u;
+            onc_u64 m = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                bool bit = (d >> b) & 1u;
+                onc_u64 bb = __ballot(bit);
+                m &= bit ? bb : ~bb;
+            }
+            onc_u32 pre = s_wc[w * 256 + d];
+            onc_u32 r = (onc_u32)__popcll(m & lt_mask);
+            rnk[k] = pre + r;
+            if (valid && r == 0u) s_wc[w * 256 + d] = pre + (onc_u32)__popcll(m);
+        }
+        __syncthreads();
+        // per digit (thread = digit): wave prefix, tile count
+        const onc_u32 c0 = s_wc[tid], c1 = s_wc[256 + tid], c2 = s_wc[512 + tid], c3 = s_wc[768 + tid];
+        const onc_u32 tot = c0 + c1 + c2 + c3;
+        s_wc[tid] = 0u;
+        s_wc[256 + tid] = c0;
+        s_wc[512 + tid] = c0 + c1;
+        s_wc[768 + tid] = c0 + c1 + c2;
+        const onc_u32 texcl = block_exscan256(tot, s_tmp, nullptr);
+        s_texcl[tid] = texcl;
+        // decoupled look-back over the sim's earlier tiles for digit tid
+        onc_u32 prefix = 0u;
+        onc_u32* mine = &status[(size_t)g * 256u + tid];
+        if (t == 0u) {
+            st_relaxed(mine, ONC_FLAG_P | tot);
+        } else {
+            st_relaxed(mine, ONC_FLAG_A | tot);
+            onc_u32 j = g - 1u, spins = 0u;
+            for (;;) {
+                onc_u32 v = ld_relaxed(&status[(size_t)j * 256u + tid]);
+                if ((v & (ONC_FLAG_A | ONC_FLAG_P)) == 0u) {
+                    if (++spins > ONC_SPIN_LIMIT) {
+                        atomicOr(a.err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += v & ONC_CNT_MASK;
+                if (v & ONC_FLAG_P) break;
+                --j;
+            }
+            st_relaxed(mine, ONC_FLAG_P | (prefix + tot));
+        }
+        const onc_u32 hd = a.hist[((size_t)s * 4u + (onc_u32)pass) * 256u + tid];
+        const onc_u32 gex = block_exscan256(hd, s_tmp, nullptr);
+        s_gbase[tid] = gex + prefix;
+        __syncthreads();
+        // reorder through LDS by digit
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            onc_u32 idx = (onc_u32)(w * 64 * ITEMS + k * 64 + lane);
+            if (idx < cnt) {
+                onc_u32 d = (key[k] >> shift) & 255u;
+                onc_u32 pos = s_texcl[d] + s_wc[w * 256 + d] + rnk[k];
+                s_keys[pos] = key[k];
+                s_vals[pos] = val[k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            onc_u32 j = (onc_u32)(k * 256 + tid);
+            if (j < cnt) {
+                onc_u32 kk = s_keys[j];
+                onc_u32 d = (kk >> shift) & 255u;
+                onc_u32 o = s_gbase[d] + (j - s_texcl[d]);
+                kout[o] = kk;
+                vout[o] = s_vals[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gather every state column into sorted order (current -> other buffer) and
+// write bin start/end of every occupied key.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
+    const onc_u32 total = a.cell_prefix[a.nsims];
+    const onc_u32 P = a.passes;
+    for (onc_u32 g = blockIdx.x * 256u + threadIdx.x; g < total; g += gridDim.x * 256u) {
+        const int s = onc_find_seg(a.cell_prefix, a.nsims, g);
+        const onc_dsim& S = a.sims[s];
+        const onc_u32 i = g - a.cell_prefix[s];
+        const onc_u32 n = S.state->n;
+        const onc_u32 par = S.state->parity;
+        const onc_u32* keys = S.keys[P & 1u];
+        const onc_u32 src = S.vals[P & 1u][i];
+        for (int c = 0; c < S.n_cols; c++) S.st[par ^ 1u][c][i] = S.st[par][c][src];
+        const onc_u32 k = keys[i];
+        if (i == 0u || keys[i - 1u] != k) S.bin_start[k] = i;
+        if (i + 1u == n || keys[i + 1u] != k) S.bin_end[k] = i + 1u;
+    }
+}
+
+// Fixed (Dirichlet) voxel predicate (engine/assets/shaders/tumor_sim.wgsl:420-423).
+__device__ __forceinline__ bool onc_is_fixed(const onc_dfield& f, const int* d, int x, int y, int z) {
+    if (f.bc == 0) return x == 0 || y == 0 || z == 0 || x == d[0] - 1 || y == d[1] - 1 || z == d[2] - 1;
+    if (z != 0) return false;
+    float fx = ((float)x + 0.5f) - (float)d[0] * 0.5f;
+    float fy = ((float)y + 0.5f) - (float)d[1] * 0.5f;
+    float r = f.src_radius;
+    return fx * fx + fy * fy <= r * r;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour forces over the 27 Moore bins (engine/assets/shaders/
+// tumor_sim.wgsl:505-628 structure; linear spring, linear adhesion taper),
+// explicit overdamped integration, and per-voxel uptake by the first cell of
+// each bin (tumor_sim.wgsl:387-412 reaction, here before the sweep).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
+    const onc_u32 total = a.cell_prefix[a.nsims];
+    const onc_u32 P = a.passes;
+    for (onc_u32 g = blockIdx.x * 256u + threadIdx.x; g < total; g += gridDim.x * 256u) {
+        const int s = onc_find_seg(a.cell_prefix, a.nsims, g);
+        const onc_dsim& S = a.sims[s];
+        const onc_u32 i = g - a.cell_prefix[s];
+        const onc_u32 n = S.state->n;
+        const onc_u32 par = S.state->parity;
+        const onc_u32* keys = S.keys[P & 1u];
+        const float* tx = (const float*)S.st[par ^ 1u][0];
+        const float* ty = (const float*)S.st[par ^ 1u][1];
+        const float* tz = (const float*)S.st[par ^ 1u][2];
+        const float px = tx[i], py = ty[i], pz = tz[i];
+        const float R = S.R, Sc = 2.0f * S.cell_radius;
+        const float R2 = R * R, RmS = R - Sc, krep = S.k_rep, kadh = S.k_adh;
+        float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
+        onc_for_neighbors(S, keys, n, px, py, pz, i, [&](onc_u32 j) {
+            float ddx = px - tx[j];
+            float ddy = py - ty[j];
+            float ddz = pz - tz[j];
+            float d2 = ddx * ddx + ddy * ddy;
+            d2 = d2 + ddz * ddz;
+            if (!(d2 < R2) || d2 == 0.0f) return;
+            float dist = __builtin_sqrtf(d2);
+            float f;
+            if (dist < Sc) {
+                f = krep * (Sc - dist);
+            } else {
+                if (kadh == 0.0f) return;
+                f = -(kadh * ((R - dist) / RmS));
+            }
+            Fx = Fx + f * (ddx / dist);
+            Fy = Fy + f * (ddy / dist);
+            Fz = Fz + f * (ddz / dist);
+        });
+        const float mob = S.mobility, md = S.max_disp;
+        float dpx = mob * Fx, dpy = mob * Fy, dpz = mob * Fz;
+        float l2 = dpx * dpx + dpy * dpy;
+        l2 = l2 + dpz * dpz;
+        if (l2 > md * md) {
+            float sc = md / __builtin_sqrtf(l2);
+            dpx = dpx * sc;
+            dpy = dpy * sc;
+            dpz = dpz * sc;
+        }
+        float* cx = (float*)S.st[par][0];
+        float* cy = (float*)S.st[par][1];
+        float* cz = (float*)S.st[par][2];
+        cx[i] = onc_clampf(px + dpx, 0.0f, (float)S.dims[0] * S.dx);
+        cy[i] = onc_clampf(py + dpy, 0.0f, (float)S.dims[1] * S.dx);
+        cz[i] = onc_clampf(pz + dpz, 0.0f, (float)S.dims[2] * S.dx);
+        // uptake, once per occupied voxel
+        const onc_u32 k = keys[i];
+        if (i == 0u || keys[i - 1u] != k) {
+            const onc_u32 cntb = S.bin_end[k] - i;
+            const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
+                      vz = onc_vox1(pz, S.dx, S.dims[2]);
+            const size_t v = (size_t)vx + (size_t)S.dims[0] * ((size_t)vy + (size_t)S.dims[1] * (size_t)vz);
+            for (int f = 0; f < S.n_fields; f++) {
+                const onc_dfield& fd = S.fields[f];
+                if (fd.uptake == 0.0f || onc_is_fixed(fd, S.dims, vx, vy, vz)) continue;
+                float* u = S.fld[par][f];
+                float w = u[v] - (float)cntb * fd.uptake;
+                u[v] = w < fd.lo ? fd.lo : w;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 7-point explicit sweep, all (sim, field) jobs in one launch. A block of
+// sbx x sby threads owns an xy tile and marches sbz planes along z, keeping
+// the z-1 / z / z+1 values of its column in registers; x/y neighbours come
+// through L1/L2. u' = clamp(u + r (sum6 - 6u) - decay u) with the oracle's
+// summation order; zero-flux faces mirror the centre.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
+    const onc_u32 b = blockIdx.x;
+    const int jb = onc_find_seg(a.jobs_prefix, a.njobs, b);
+    const onc_u32 job = a.jobs[jb];
+    const onc_dsim& S = a.sims[job >> 8];
+    const int f = (int)(job & 255u);
+    const int X = S.dims[0], Y = S.dims[1], Z = S.dims[2];
+    const int sbx = a.sbx, sby = a.sby, sbz = a.sbz;
+    const onc_u32 local = b - a.jobs_prefix[jb];
+    const onc_u32 nbx = (onc_u32)((X + sbx - 1) / sbx), nby = (onc_u32)((Y + sby - 1) / sby);
+    const int bx = (int)(local % nbx);
+    const onc_u32 rest = local / nbx;
+    const int by = (int)(rest % nby);
+    const int bz = (int)(rest / nby);
+    const int x = bx * sbx + (int)(threadIdx.x % (unsigned)sbx);
+    const int y = by * sby + (int)(threadIdx.x / (unsigned)sbx);
+    if (x >= X || y >= Y) return;
+    const int z0 = bz * sbz;
+    const int z1 = min(Z, z0 + sbz);
+    const onc_u32 par = S.state->parity;
+    const float* __restrict__ in = S.fld[par][f];
+    float* __restrict__ out = S.fld[par ^ 1u][f];
+    const onc_dfield fd = S.fields[f];
+    const size_t XY = (size_t)X * (size_t)Y;
+    const size_t col = (size_t)x + (size_t)X * (size_t)y;
+    float cc = in[col + XY * (size_t)z0];
+    float cm = z0 > 0 ? in[col + XY * (size_t)(z0 - 1)] : cc;
+    for (int z = z0; z < z1; z++) {
+        const size_t v = col + XY * (size_t)z;
+        const float cp = (z + 1 < Z) ? in[v + XY] : cc;
+        if (onc_is_fixed(fd, S.dims, x, y, z)) {
+            out[v] = fd.bc_value;
+        } else {
+            const float xm = x > 0 ? in[v - 1] : cc;
+            const float xp = x < X - 1 ? in[v + 1] : cc;
+            const float ym = y > 0 ? in[v - (size_t)X] : cc;
+            const float yp = y < Y - 1 ? in[v + (size_t)X] : cc;
+            const float zm = z > 0 ? cm : cc;
+            const float zp = z + 1 < Z ? cp : cc;
+            float sum = xm + xp;
+            sum = sum + ym;
+            sum = sum + yp;
+            sum = sum + zm;
+            sum = sum + zp;
+            const float lap = sum - 6.0f * cc;
+            float w = cc + fd.rate * lap;
+            w = w - fd.decay * cc;
+            out[v] = onc_clampf(w, fd.lo, fd.hi);
+        }
+        cm = cc;
+        cc = cp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Built-in tumour rule (the BASELINE model). Reads the permuted state (other
+// buffer), the integrated position and the swept fields; writes the cell's
+// columns to the current buffer, its alive/divide flags and its daughter.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
+    const onc_u32 total = r.cell_prefix[r.nsims];
+    for (onc_u32 g = blockIdx.x * 256u + threadIdx.x; g < total; g += gridDim.x * 256u) {
+        const int s = onc_find_seg(r.cell_prefix, r.nsims, g);
+        const onc_dsim& S = r.sims[s];
+        const onc_u32 i = g - r.cell_prefix[s];
+        const onc_u32 par = S.state->parity;
+        onc_u32* const* T = S.st[par ^ 1u];
+        onc_u32* const* C = S.st[par];
+        float px = ((const float*)C[0])[i], py = ((const float*)C[1])[i], pz = ((const float*)C[2])[i];
+        const onc_u32 id = T[3][i];
+        float age = __uint_as_float(T[4][i]);
+        onc_u32 gen = T[5][i];
+        const onc_u32 v = onc_field_index(S, px, py, pz);
+        const float o2 = S.fld[par ^ 1u][0][v];
+        onc_rng rng;
+        onc_rng_init(rng, S, id, S.state->step);
+        age = age + 1.0f;
+        onc_u32 alive = 1u, div = 0u;
+        float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
+        if (o2 < S.params[2]) {
+            alive = 0u;
+        } else {
+            if (S.n_fields > 1 && S.params[3] > 0.0f) {
+                const float drug = S.fld[par ^ 1u][1][v];
+                const float u = onc_rand(rng);
+                if (u < S.params[3] * drug) alive = 0u;
+            }
+            if (alive && age > S.params[0] && o2 > S.params[1]) {
+                float ux, uy, uz;
+                onc_rand_dir(rng, &ux, &uy, &uz);
+                const float ofs = S.params[4];
+                dpx = onc_clampf(px + ux * ofs, 0.0f, (float)S.dims[0] * S.dx);
+                dpy = onc_clampf(py + uy * ofs, 0.0f, (float)S.dims[1] * S.dx);
+                dpz = onc_clampf(pz + uz * ofs, 0.0f, (float)S.dims[2] * S.dx);
+                age = 0.0f;
+                gen = gen + 1u;
+                div = 1u;
+            }
+        }
+        C[3][i] = id;
+        C[4][i] = __float_as_uint(age);
+        C[5][i] = gen;
+        for (int c = 6; c < S.n_cols; c++) C[c][i] = T[c][i];
+        S.flags[i] = alive | (div << 1);
+        if (div) {
+            S.dau[0][i] = __float_as_uint(dpx);
+            S.dau[1][i] = __float_as_uint(dpy);
+            S.dau[2][i] = __float_as_uint(dpz);
+            S.dau[3][i] = 0u;
+            S.dau[4][i] = __float_as_uint(0.0f);
+            S.dau[5][i] = gen;
+            for (int c = 6; c < S.n_cols; c++) S.dau[c][i] = T[c][i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compaction: scan of (alive, divide) flags with decoupled look-back;
+// survivors are gathered in order into the other buffer, daughters into a
+// dense staging area in parent order. The sim's last tile records totals.
+// ---------------------------------------------------------------------------
+#define ONC_F64_A (1ull << 62)
+#define ONC_F64_P (2ull << 62)
+#define ONC_M31 0x7FFFFFFFull
+
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
+    constexpr int TILE = 256 * ITEMS;
+    constexpr onc_u32 NONE = 0xFFFFFFFFu;
+    __shared__ onc_u32 s_aslot[TILE];
+    __shared__ onc_u32 s_dslot[TILE];
+    __shared__ onc_u32 s_tmp[8];
+    __shared__ onc_u32 s_tile, s_pa, s_pd;
+    const int tid = threadIdx.x;
+    const onc_u32 total = a.tile_prefix[a.nsims];
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(&a.counters[ONC_CTR_COMPACT], 1u);
+        __syncthreads();
+        const onc_u32 g = s_tile;
+        if (g >= total) break;
+        const int s = onc_find_seg(a.tile_prefix, a.nsims, g);
+        const onc_dsim& S = a.sims[s];
+        const onc_u32 t = g - a.tile_prefix[s];
+        const onc_u32 ntiles_s = a.tile_prefix[s + 1] - a.tile_prefix[s];
+        const onc_u32 n = S.state->n;
+        const onc_u32 base = t * (onc_u32)TILE;
+        const onc_u32 cnt = min((onc_u32)TILE, n - base);
+        const onc_u32 par = S.state->parity;
+        // flags, striped load into LDS
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            onc_u32 j = (onc_u32)(k * 256 + tid);
+            s_aslot[j] = j < cnt ? S.flags[base + j] : 0u;
+        }
+        __syncthreads();
+        onc_u32 la = 0u, ldv = 0u;
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            onc_u32 f = s_aslot[tid * ITEMS + k];
+            la += f & 1u;
+            ldv += (f >> 1) & 1u;
+        }
+        onc_u32 tp;
+        const onc_u32 ex = block_exscan256(la | (ldv << 16), s_tmp, &tp);
+        const onc_u32 ta = tp & 0xFFFFu, td = tp >> 16;
+        if (tid == 0) {
+            onc_u64 pa = 0ull, pd = 0ull;
+            onc_u64* mine = &a.comp_status[g];
+            const onc_u64 agg = (onc_u64)ta | ((onc_u64)td << 31);
+            if (t == 0u) {
+                st_relaxed64(mine, ONC_F64_P | agg);
+            } else {
+                st_relaxed64(mine, ONC_F64_A | agg);
+                onc_u32 j = g - 1u, spins = 0u;
+                for (;;) {
+                    onc_u64 v = ld_relaxed64(&a.comp_status[j]);
+                    if ((v >> 62) == 0ull) {
+                        if (++spins > ONC_SPIN_LIMIT) {
+                            atomicOr(a.err, 2u);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    pa += v & ONC_M31;
+                    pd += (v >> 31) & ONC_M31;
+                    if ((v >> 62) == 2ull) break;
+                    --j;
+                }
+                st_relaxed64(mine, ONC_F64_P | ((pa + ta) | ((pd + td) << 31)));
+            }
+            s_pa = (onc_u32)pa;
+            s_pd = (onc_u32)pd;
+            if (t + 1u == ntiles_s) {
+                S.state->nsurv = (onc_u32)pa + ta;
+                S.state->ndiv = (onc_u32)pd + td;
+            }
+        }
+        __syncthreads();
+        onc_u32 ra = s_pa + (ex & 0xFFFFu), rd = s_pd + (ex >> 16);
+        onc_u32 fl[ITEMS];
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) fl[k] = s_aslot[tid * ITEMS + k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const onc_u32 f = fl[k];
+            s_aslot[tid * ITEMS + k] = (f & 1u) ? ra++ : NONE;
+            s_dslot[tid * ITEMS + k] = (f & 2u) ? rd++ : NONE;
+        }
+        __syncthreads();
+        onc_u32* const* C = S.st[par];
+        onc_u32* const* T = S.st[par ^ 1u];
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const onc_u32 j = (onc_u32)(k * 256 + tid);
+            if (j < cnt) {
+                const onc_u32 i = base + j;
+                const onc_u32 sa = s_aslot[j];
+                if (sa != NONE)
+                    for (int c = 0; c < S.n_cols; c++) T[c][sa] = C[c][i];
+                const onc_u32 sd = s_dslot[j];
+                if (sd != NONE)
+                    for (int c = 0; c < S.n_cols; c++) S.dau2[c][sd] = S.dau[c][i];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Append daughters after the survivors (capacity-clamped), assign ids, and
+// record the counters the next plan commits.
+__global__ __launch_bounds__(256) void k_daughters(onc_kargs a) {
+    const onc_u32 total = a.tile_prefix[a.nsims];
+    for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
+        const int s = onc_find_seg(a.tile_prefix, a.nsims, g);
+        const onc_dsim& S = a.sims[s];
+        const onc_u32 t = g - a.tile_prefix[s];
+        const onc_u32 nsurv = S.state->nsurv, ndiv = S.state->ndiv;
+        const onc_u32 room = S.cap - nsurv;
+        const onc_u32 avail = ndiv < room ? ndiv : room;
+        const onc_u32 nid = S.state->next_id;
+        const onc_u32 par = S.state->parity;
+        onc_u32* const* T = S.st[par ^ 1u];
+        for (onc_u32 j = threadIdx.x; j < a.tile; j += 256u) {
+            const onc_u32 k = t * a.tile + j;
+            if (k < avail) {
+                for (int c = 0; c < S.n_cols; c++) T[c][nsurv + k] = (c == 3) ? nid + k : S.dau2[c][k];
+            }
+        }
+        if (t == 0u && threadIdx.x == 0) {
+            S.state->pend_n = nsurv + avail;
+            S.state->pend_next_id = nid + avail;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pure kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_morton_keys(const float* px, const float* py, const float* pz, onc_u32 n,
+                                                    int X, int Y, int Z, float dx, int fmt, onc_u32* keys) {
+    for (onc_u32 i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        keys[i] = onc_morton((onc_u32)onc_vox1(px[i], dx, X), (onc_u32)onc_vox1(py[i], dx, Y),
+                             (onc_u32)onc_vox1(pz[i], dx, Z), fmt);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bins(const onc_u32* keys, onc_u32 n, onc_u32* start, onc_u32* end) {
+    for (onc_u32 i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const onc_u32 k = keys[i];
+        if (i == 0u || keys[i - 1u] != k) start[k] = i;
+        if (i + 1u == n || keys[i + 1u] != k) end[k] = i + 1u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_philox(const onc_u32* ctr, onc_u32 n, onc_u32 k0, onc_u32 k1,
+                                               onc_u32* out) {
+    for (onc_u32 i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        onc_u32 o[4];
+        onc_philox(ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3], k0, k1, o);
+        out[4 * i] = o[0];
+        out[4 * i + 1] = o[1];
+        out[4 * i + 2] = o[2];
+        out[4 * i + 3] = o[3];
+    }
+}
+
+// Field initial state: init everywhere, bc_value on fixed voxels.
+__global__ __launch_bounds__(256) void k_field_init(float* u, int X, int Y, int Z, onc_dfield f) {
+    const size_t nv = (size_t)X * (size_t)Y * (size_t)Z;
+    const int d[3] = {X, Y, Z};
+    for (size_t v = (size_t)blockIdx.x * 256u + threadIdx.x; v < nv; v += (size_t)gridDim.x * 256u) {
+        const int x = (int)(v % (size_t)X);
+        const int y = (int)((v / (size_t)X) % (size_t)Y);
+        const int z = (int)(v / ((size_t)X * (size_t)Y));
+        u[v] = onc_is_fixed(f, d, x, y, z) ? f.bc_value : f.init;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy_u32(const onc_u32* __restrict__ src, onc_u32* __restrict__ dst,
+                                                 size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (size_t)gridDim.x * 256u) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(onc_u32* p, onc_u32 v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (size_t)gridDim.x * 256u) p[i] = v;
+}
+
+// Standalone exclusive scan: single pass, decoupled look-back, 64-bit status
+// words (2 flag bits, 32-bit running sum).
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_exscan(const onc_u32* __restrict__ in, onc_u32* __restrict__ out,
+                                                onc_u32 n, onc_u32* total, onc_u64* status, onc_u32* counter,
+                                                onc_u32* err) {
+    constexpr int TILE = 256 * ITEMS;
+    __shared__ onc_u32 s_v[TILE];
+    __shared__ onc_u32 s_tmp[8];
+    __shared__ onc_u32 s_tile, s_pre;
+    const int tid = threadIdx.x;
+    const onc_u32 ntiles = (n + TILE - 1) / TILE;
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(counter, 1u);
+        __syncthreads();
+        const onc_u32 g = s_tile;
+        if (g >= ntiles) break;
+        const onc_u32 base = g * (onc_u32)TILE;
+        const onc_u32 cnt = min((onc_u32)TILE, n - base);
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            onc_u32 j = (onc_u32)(k * 256 + tid);
+            s_v[j] = j < cnt ? in[base + j] : 0u;
+        }
+        __syncthreads();
+        onc_u32 v[ITEMS];
+        onc_u32 sum = 0u;
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            v[k] = s_v[tid * ITEMS + k];
+            sum += v[k];
+        }
+        onc_u32 tot;
+        const onc_u32 ex = block_exscan256(sum, s_tmp, &tot);
+        if (tid == 0) {
+            onc_u32 pre = 0u;
+            onc_u64* mine = &status[g];
+            if (g == 0u) {
+                st_relaxed64(mine, ONC_F64_P | (onc_u64)tot);
+            } else {
+                st_relaxed64(mine, ONC_F64_A | (onc_u64)tot);
+                onc_u32 j = g - 1u, spins = 0u;
+                for (;;) {
+                    onc_u64 w = ld_relaxed64(&status[j]);
+                    if ((w >> 62) == 0ull) {
+                        if (++spins > ONC_SPIN_LIMIT) {
+                            atomicOr(err, 4u);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    pre += (onc_u32)w;
+                    if ((w >> 62) == 2ull) break;
+                    --j;
+                }
+                st_relaxed64(mine, ONC_F64_P | (onc_u64)(pre + tot));
+            }
+            s_pre = pre;
+            if (g + 1u == ntiles && total) *total = pre + tot;
+        }
+        __syncthreads();
+        onc_u32 run = s_pre + ex;
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            s_v[tid * ITEMS + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            onc_u32 j = (onc_u32)(k * 256 + tid);
+            if (j < cnt) out[base + j] = s_v[j];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, s, a, mode);
+    return hipGetLastError();
+}
+
+#define ONC_DISPATCH_ITEMS(items, KERNEL, grid, s, ...)                                      \
+    do {                                                                                     \
+        if ((items) == 8)                                                                    \
+            hipLaunchKernelGGL(KERNEL<8>, dim3(grid), dim3(256), 0, s, __VA_ARGS__);        \
+        else if ((items) == 12)                                                              \
+            hipLaunchKernelGGL(KERNEL<12>, dim3(grid), dim3(256), 0, s, __VA_ARGS__);       \
+        else                                                                                 \
+            hipLaunchKernelGGL(KERNEL<16>, dim3(grid), dim3(256), 0, s, __VA_ARGS__);       \
+    } while (0)
+
+hipError_t onc_launch_morton(const onc_kargs& a, int items, int grid, hipStream_t s) {
+    ONC_DISPATCH_ITEMS(items, k_morton, grid, s, a);
+    return hipGetLastError();
+}
+hipError_t onc_launch_hist_only(const onc_kargs& a, int items, int grid, hipStream_t s) {
+    ONC_DISPATCH_ITEMS(items, k_hist_only, grid, s, a);
+    return hipGetLastError();
+}
+hipError_t onc_launch_sort_pass(const onc_kargs& a, int items, int pass, int grid, hipStream_t s) {
+    ONC_DISPATCH_ITEMS(items, k_sort_pass, grid, s, a, pass);
+    return hipGetLastError();
+}
+hipError_t onc_launch_permute_bins(const onc_kargs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_permute_bins, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t onc_launch_forces(const onc_kargs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_forces, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t onc_launch_stencil(const onc_kargs& a, onc_u32 nblocks, hipStream_t s) {
+    if (nblocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stencil, dim3(nblocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t onc_launch_rule_builtin(const onc_rule_args& r, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_rule_tumor, dim3(grid), dim3(256), 0, s, r);
+    return hipGetLastError();
+}
+hipError_t onc_launch_compact(const onc_kargs& a, int items, int grid, hipStream_t s) {
+    ONC_DISPATCH_ITEMS(items, k_compact, grid, s, a);
+    return hipGetLastError();
+}
+hipError_t onc_launch_daughters(const onc_kargs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_daughters, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+static int grid_for(size_t n) {
+    size_t b = (n + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    return (int)b;
+}
+
+hipError_t onc_launch_morton_keys(const float* px, const float* py, const float* pz, onc_u32 n, int X, int Y,
+                                  int Z, float dx, int fmt, onc_u32* keys, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_morton_keys, dim3(grid_for(n)), dim3(256), 0, s, px, py, pz, n, X, Y, Z, dx, fmt, keys);
+    return hipGetLastError();
+}
+hipError_t onc_launch_bins(const onc_u32* keys, onc_u32 n, onc_u32* start, onc_u32* end, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bins, dim3(grid_for(n)), dim3(256), 0, s, keys, n, start, end);
+    return hipGetLastError();
+}
+hipError_t onc_launch_philox(const onc_u32* ctr, onc_u32 n, onc_u32 k0, onc_u32 k1, onc_u32* out,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_philox, dim3(grid_for(n)), dim3(256), 0, s, ctr, n, k0, k1, out);
+    return hipGetLastError();
+}
+hipError_t onc_launch_exscan(const onc_u32* in, onc_u32* out, onc_u32 n, onc_u32* total, onc_u32* status,
+                             onc_u32* counter, onc_u32* err, int grid, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_exscan<16>, dim3(grid), dim3(256), 0, s, in, out, n, total, (onc_u64*)status, counter,
+                       err);
+    return hipGetLastError();
+}
+hipError_t onc_launch_fill_u32(onc_u32* p, onc_u32 v, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+hipError_t onc_launch_field_init(float* u, int X, int Y, int Z, const onc_dfield& f, hipStream_t s) {
+    const size_t nv = (size_t)X * (size_t)Y * (size_t)Z;
+    if (nv == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_field_init, dim3(grid_for(nv)), dim3(256), 0, s, u, X, Y, Z, f);
+    return hipGetLastError();
+}
+hipError_t onc_launch_copy_u32(const onc_u32* src, onc_u32* dst, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}

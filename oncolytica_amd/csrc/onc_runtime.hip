@@ -1,0 +1,1120 @@
+// onc_runtime.hip — the C ABI of liboncohip.so (include/onco.h): device
+// contexts, sims and their device buffers, batches over a packed segment
+// table with an optional captured hipGraph per timestep, hiprtc rule modules,
+// and the pure-kernel entry points.
+//
+// Host orchestration replaces GpuBackend.run_step / _dispatch_sort_pipeline
+// (framework/src/oncolytica/gpu/runtime/_gpu_backend.py:113-171, 407-432):
+// one HIP stream per batch, one launch per phase for all of its sims, live
+// counts kept on the device.
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/onco.h"
+#include "onc_device_src.h"  // generated: onc_device_h_src (text of onc_device.h)
+#include "onc_internal.h"
+
+static_assert(sizeof(onc_field_desc) == sizeof(onc_dfield), "field desc layout");
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define CK(call)                                                                                   \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ONC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+#define CKR(expr)              \
+    do {                       \
+        int r_ = (expr);       \
+        if (r_ != ONC_OK) return r_; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct onc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    std::string arch;
+    hipDeviceProp_t prop{};
+    // scratch for pure kernels
+    std::vector<void*> scratch;
+    size_t scratch_bytes = 0;
+};
+
+struct onc_module {
+    onc_ctx* ctx = nullptr;
+    hipModule_t mod = nullptr;
+    std::string log;
+};
+
+struct onc_batch;
+
+struct onc_sim {
+    onc_ctx* ctx = nullptr;
+    onc_sim_desc desc{};
+    onc_dsim dsim{};        // host copy with device pointers
+    onc_dstate* dstate = nullptr;
+    std::vector<void*> allocs;
+    uint64_t footprint = 0;
+    size_t nvox = 0;
+    uint32_t table = 0;     // bin table entries
+    int key_bits = 0;
+    hipFunction_t rule_fn = nullptr;  // nullptr: built-in tumour rule
+    onc_module* rule_mod = nullptr;
+    onc_batch* owner = nullptr;
+    onc_batch* solo = nullptr;
+};
+
+struct onc_batch {
+    onc_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<onc_sim*> sims;
+    bool is_solo = false;
+    bool dirty = true;
+    // device tables
+    onc_dsim* d_sims = nullptr;
+    onc_u32* d_small = nullptr;  // prefixes, counters, err
+    onc_u32* d_hist = nullptr;
+    onc_u32* d_sort_status = nullptr;
+    onc_u64* d_comp_status = nullptr;
+    onc_u32* d_jobs = nullptr;
+    size_t cap_sims = 0, cap_tiles = 0, cap_jobs = 0;
+    onc_kargs ka{};
+    onc_rule_args ra{};
+    onc_u32 stencil_blocks = 0;
+    uint64_t total_cap = 0;
+    int passes = 1;
+    // tuning
+    int items = 16;
+    int sbx = 64, sby = 4, sbz = 16;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // profiling
+    bool profile = false;
+    std::vector<hipEvent_t> ev;
+    double phase_us[ONC_PHASES] = {0};
+    uint64_t phase_n[ONC_PHASES] = {0};
+};
+
+// ---------------------------------------------------------------------------
+// errors / version
+// ---------------------------------------------------------------------------
+extern "C" const char* onc_last_error(void) { return g_err.c_str(); }
+extern "C" int onc_abi_version(void) { return ONC_ABI_VERSION; }
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int onc_ctx_create(int device, onc_ctx** out) {
+    if (!out) return fail(ONC_ERR_INVALID, "out is NULL");
+    int n = 0;
+    CK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(ONC_ERR_INVALID, "no such device");
+    CK(hipSetDevice(device));
+    auto* c = new onc_ctx();
+    c->device = device;
+    hipError_t e = hipGetDeviceProperties(&c->prop, device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(ONC_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    }
+    c->cus = c->prop.multiProcessorCount > 0 ? c->prop.multiProcessorCount : 256;
+    c->arch = c->prop.gcnArchName;
+    size_t colon = c->arch.find(':');
+    if (colon != std::string::npos) c->arch = c->arch.substr(0, colon);
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(ONC_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return ONC_OK;
+}
+
+extern "C" int onc_ctx_destroy(onc_ctx* c) {
+    if (!c) return ONC_OK;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (void* p : c->scratch) hipFree(p);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return ONC_OK;
+}
+
+extern "C" int onc_ctx_info(onc_ctx* c, onc_device_info* out) {
+    if (!c || !out) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(c->device));
+    memset(out, 0, sizeof(*out));
+    snprintf(out->name, sizeof(out->name), "%s", c->prop.name);
+    snprintf(out->arch, sizeof(out->arch), "%s", c->arch.c_str());
+    out->compute_units = c->cus;
+    out->xcds = c->cus >= 256 ? 8 : 1;
+    size_t fr = 0, tot = 0;
+    CK(hipMemGetInfo(&fr, &tot));
+    out->hbm_bytes = tot;
+    out->free_bytes = fr;
+    return ONC_OK;
+}
+
+extern "C" int onc_dev_alloc(onc_ctx* c, size_t bytes, void** out) {
+    if (!c || !out) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(c->device));
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 4);
+    if (e != hipSuccess) return fail(ONC_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    *out = p;
+    return ONC_OK;
+}
+extern "C" int onc_dev_free(onc_ctx* c, void* p) {
+    if (!c) return fail(ONC_ERR_INVALID, "NULL ctx");
+    CK(hipSetDevice(c->device));
+    CK(hipFree(p));
+    return ONC_OK;
+}
+extern "C" int onc_h2d(onc_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return fail(ONC_ERR_INVALID, "NULL ctx");
+    if (!bytes) return ONC_OK;
+    CK(hipSetDevice(c->device));
+    CK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return ONC_OK;
+}
+extern "C" int onc_d2h(onc_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return fail(ONC_ERR_INVALID, "NULL ctx");
+    if (!bytes) return ONC_OK;
+    CK(hipSetDevice(c->device));
+    CK(hipStreamSynchronize(c->stream));
+    CK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return ONC_OK;
+}
+extern "C" int onc_ctx_sync(onc_ctx* c) {
+    if (!c) return fail(ONC_ERR_INVALID, "NULL ctx");
+    CK(hipSetDevice(c->device));
+    CK(hipStreamSynchronize(c->stream));
+    return ONC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// rule modules (hiprtc)
+// ---------------------------------------------------------------------------
+extern "C" int onc_module_build(onc_ctx* c, const char* src, const char* const* opts, int n_opts,
+                                onc_module** out) {
+    if (!c || !src || !out) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(c->device));
+    hiprtcProgram prog;
+    const char* hdrs[1] = {onc_device_h_src};
+    const char* names[1] = {"onc_device.h"};
+    if (hiprtcCreateProgram(&prog, src, "onc_rule.hip", 1, hdrs, names) != HIPRTC_SUCCESS)
+        return fail(ONC_ERR_COMPILE, "hiprtcCreateProgram failed");
+    std::string arch = "--offload-arch=" + c->arch;
+    std::vector<const char*> o = {arch.c_str(), "-O3", "-ffp-contract=off", "-std=c++17",
+                                  "-fhip-fp32-correctly-rounded-divide-sqrt",
+                                  "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature",
+                                  "-Xclang", "-packed-fp32-ops"};
+    for (int i = 0; i < n_opts; i++) o.push_back(opts[i]);
+    hiprtcResult rc = hiprtcCompileProgram(prog, (int)o.size(), o.data());
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &log[0]);
+    if (rc != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return fail(ONC_ERR_COMPILE, "hiprtc compilation failed:\n" + log);
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    std::vector<char> code(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    auto* m = new onc_module();
+    m->ctx = c;
+    m->log = log;
+    hipError_t e = hipModuleLoadData(&m->mod, code.data());
+    if (e != hipSuccess) {
+        delete m;
+        return fail(ONC_ERR_COMPILE, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
+    }
+    *out = m;
+    return ONC_OK;
+}
+
+extern "C" int onc_module_log(onc_module* m, char* buf, size_t len) {
+    if (!m || !buf || !len) return fail(ONC_ERR_INVALID, "NULL argument");
+    snprintf(buf, len, "%s", m->log.c_str());
+    return ONC_OK;
+}
+
+extern "C" int onc_module_destroy(onc_module* m) {
+    if (!m) return ONC_OK;
+    hipSetDevice(m->ctx->device);
+    hipModuleUnload(m->mod);
+    delete m;
+    return ONC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sims
+// ---------------------------------------------------------------------------
+static uint32_t host_morton(uint32_t x, uint32_t y, uint32_t z, int fmt) {
+    uint32_t k = 0;
+    for (int b = 0; b < 10; b++) {
+        k |= ((x >> b) & 1u) << (3 * b);
+        k |= ((y >> b) & 1u) << (3 * b + 1);
+        k |= ((z >> b) & 1u) << (3 * b + 2);
+    }
+    if (fmt == ONC_KEY_MORTON32) k |= (((x >> 10) & 1u) << 30) | (((y >> 10) & 1u) << 31);
+    return k;
+}
+
+static int validate_desc(const onc_sim_desc* d) {
+    if (!d) return fail(ONC_ERR_INVALID, "desc is NULL");
+    if (d->key_format != ONC_KEY_MORTON30 && d->key_format != ONC_KEY_MORTON32)
+        return fail(ONC_ERR_INVALID, "key_format must be 30 or 32");
+    int lim_xy = d->key_format == ONC_KEY_MORTON32 ? 2048 : 1024;
+    for (int i = 0; i < 3; i++) {
+        int lim = i < 2 ? lim_xy : 1024;
+        if (d->dims[i] < 1 || d->dims[i] > lim) return fail(ONC_ERR_INVALID, "dims out of range for key format");
+    }
+    if (!(d->dx > 0.0f)) return fail(ONC_ERR_INVALID, "dx must be > 0");
+    if (d->capacity < 1 || d->capacity > (1u << 30)) return fail(ONC_ERR_INVALID, "capacity must be in [1, 2^30]");
+    if (d->n_cols < 4 || d->n_cols > ONC_MAX_COLS) return fail(ONC_ERR_INVALID, "n_cols must be in [4, 48]");
+    for (int c = 0; c < 3; c++)
+        if (d->col_types[c] != ONC_COL_F32) return fail(ONC_ERR_INVALID, "columns 0..2 (pos) must be f32");
+    if (d->col_types[3] != ONC_COL_U32) return fail(ONC_ERR_INVALID, "column 3 (id) must be u32");
+    if (d->n_fields < 0 || d->n_fields > ONC_MAX_FIELDS) return fail(ONC_ERR_INVALID, "n_fields must be in [0, 4]");
+    if (d->n_params < 0 || d->n_params > ONC_MAX_PARAMS) return fail(ONC_ERR_INVALID, "n_params must be in [0, 32]");
+    if (d->interaction_radius > d->dx) return fail(ONC_ERR_INVALID, "interaction_radius must be <= dx (bins are voxels)");
+    if (d->interaction_radius < 0.0f || d->cell_radius < 0.0f || d->max_disp < 0.0f)
+        return fail(ONC_ERR_INVALID, "negative mechanics parameter");
+    for (int f = 0; f < d->n_fields; f++) {
+        const onc_field_desc& fd = d->fields[f];
+        if (fd.bc != ONC_BC_DIRICHLET && fd.bc != ONC_BC_POINT_SOURCE) return fail(ONC_ERR_INVALID, "bad field bc");
+        if (!(fd.lo <= fd.hi)) return fail(ONC_ERR_INVALID, "field lo > hi");
+    }
+    return ONC_OK;
+}
+
+static void sim_layout(const onc_sim_desc* d, uint32_t* table, int* key_bits, size_t* nvox, uint64_t* bytes) {
+    uint32_t maxkey = host_morton((uint32_t)d->dims[0] - 1, (uint32_t)d->dims[1] - 1, (uint32_t)d->dims[2] - 1,
+                                  d->key_format);
+    int kb = 1;
+    while (kb < 32 && (maxkey >> kb) != 0u) kb++;
+    *key_bits = kb;
+    *table = maxkey + 1u;
+    *nvox = (size_t)d->dims[0] * (size_t)d->dims[1] * (size_t)d->dims[2];
+    uint64_t cap = d->capacity;
+    uint64_t b = 0;
+    b += 2ull * d->n_cols * cap * 4;  // state, two buffers
+    b += 2ull * d->n_cols * cap * 4;  // daughters (sparse + compacted)
+    b += 5ull * cap * 4;              // keys x2, vals x2, flags
+    b += 2ull * d->n_fields * (*nvox) * 4;
+    b += 2ull * (uint64_t)(*table) * 4;
+    b += 256;
+    *bytes = b;
+}
+
+extern "C" int onc_sim_footprint(const onc_sim_desc* d, uint64_t* bytes) {
+    CKR(validate_desc(d));
+    if (!bytes) return fail(ONC_ERR_INVALID, "bytes is NULL");
+    uint32_t t;
+    int kb;
+    size_t nv;
+    sim_layout(d, &t, &kb, &nv, bytes);
+    return ONC_OK;
+}
+
+static int sim_alloc(onc_sim* s, size_t bytes, void** out) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 4);
+    if (e != hipSuccess) return fail(ONC_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    s->allocs.push_back(p);
+    *out = p;
+    return ONC_OK;
+}
+
+static void fill_dsim_static(onc_sim* s) {
+    const onc_sim_desc& d = s->desc;
+    onc_dsim& D = s->dsim;
+    for (int i = 0; i < 3; i++) D.dims[i] = d.dims[i];
+    D.dx = d.dx;
+    D.key_format = d.key_format;
+    D.key_bits = s->key_bits;
+    D.cap = d.capacity;
+    D.seed_lo = (onc_u32)d.seed;
+    D.seed_hi = (onc_u32)(d.seed >> 32);
+    D.cell_radius = d.cell_radius;
+    D.R = d.interaction_radius;
+    D.k_rep = d.k_rep;
+    D.k_adh = d.k_adh;
+    D.mobility = d.mobility;
+    D.max_disp = d.max_disp;
+    D.n_fields = d.n_fields;
+    D.n_cols = d.n_cols;
+    memcpy(D.fields, d.fields, sizeof(D.fields));
+    memcpy(D.params, d.params, sizeof(D.params));
+}
+
+static int batch_create_internal(onc_ctx* c, bool solo, onc_batch** out);
+static int batch_commit(onc_batch* b);
+
+extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) {
+    if (!c || !out) return fail(ONC_ERR_INVALID, "NULL argument");
+    CKR(validate_desc(d));
+    CK(hipSetDevice(c->device));
+    auto* s = new onc_sim();
+    s->ctx = c;
+    s->desc = *d;
+    sim_layout(d, &s->table, &s->key_bits, &s->nvox, &s->footprint);
+    fill_dsim_static(s);
+    auto cleanup = [&](int rc) {
+        for (void* p : s->allocs) hipFree(p);
+        delete s;
+        return rc;
+    };
+    const size_t cap = d->capacity;
+    void* p = nullptr;
+    int rc;
+    if ((rc = sim_alloc(s, 2ull * d->n_cols * cap * 4, &p)) != ONC_OK) return cleanup(rc);
+    for (int b = 0; b < 2; b++)
+        for (int col = 0; col < d->n_cols; col++)
+            s->dsim.st[b][col] = (onc_u32*)p + ((size_t)b * d->n_cols + col) * cap;
+    if ((rc = sim_alloc(s, 2ull * d->n_cols * cap * 4, &p)) != ONC_OK) return cleanup(rc);
+    for (int col = 0; col < d->n_cols; col++) {
+        s->dsim.dau[col] = (onc_u32*)p + (size_t)col * cap;
+        s->dsim.dau2[col] = (onc_u32*)p + ((size_t)d->n_cols + col) * cap;
+    }
+    if ((rc = sim_alloc(s, 5ull * cap * 4, &p)) != ONC_OK) return cleanup(rc);
+    s->dsim.keys[0] = (onc_u32*)p;
+    s->dsim.keys[1] = (onc_u32*)p + cap;
+    s->dsim.vals[0] = (onc_u32*)p + 2 * cap;
+    s->dsim.vals[1] = (onc_u32*)p + 3 * cap;
+    s->dsim.flags = (onc_u32*)p + 4 * cap;
+    if (d->n_fields > 0) {
+        if ((rc = sim_alloc(s, 2ull * d->n_fields * s->nvox * 4, &p)) != ONC_OK) return cleanup(rc);
+        for (int b = 0; b < 2; b++)
+            for (int f = 0; f < d->n_fields; f++)
+                s->dsim.fld[b][f] = (float*)p + ((size_t)b * d->n_fields + f) * s->nvox;
+    }
+    if ((rc = sim_alloc(s, 2ull * s->table * 4, &p)) != ONC_OK) return cleanup(rc);
+    s->dsim.bin_start = (onc_u32*)p;
+    s->dsim.bin_end = (onc_u32*)p + s->table;
+    if ((rc = sim_alloc(s, sizeof(onc_dstate), &p)) != ONC_OK) return cleanup(rc);
+    s->dstate = (onc_dstate*)p;
+    s->dsim.state = s->dstate;
+    // initial device state
+    hipStream_t st = c->stream;
+    onc_dstate z{};
+    hipError_t e = hipMemcpyAsync(s->dstate, &z, sizeof(z), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = onc_launch_fill_u32(s->dsim.bin_start, 0xFFFFFFFFu, 2ull * s->table, st);
+    for (int f = 0; f < d->n_fields && e == hipSuccess; f++) {
+        onc_dfield fd;
+        memcpy(&fd, &d->fields[f], sizeof(fd));
+        e = onc_launch_field_init(s->dsim.fld[0][f], d->dims[0], d->dims[1], d->dims[2], fd, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return cleanup(fail(ONC_ERR_HIP, std::string("sim init: ") + hipGetErrorString(e)));
+    if ((rc = batch_create_internal(c, true, &s->solo)) != ONC_OK) return cleanup(rc);
+    s->solo->sims.push_back(s);
+    s->owner = s->solo;
+    *out = s;
+    return ONC_OK;
+}
+
+static int batch_destroy_internal(onc_batch* b);
+
+extern "C" int onc_sim_destroy(onc_sim* s) {
+    if (!s) return ONC_OK;
+    hipSetDevice(s->ctx->device);
+    if (s->owner && !s->owner->is_solo) {
+        onc_batch* b = s->owner;
+        hipStreamSynchronize(b->stream);
+        b->sims.erase(std::remove(b->sims.begin(), b->sims.end(), s), b->sims.end());
+        b->dirty = true;
+    }
+    if (s->solo) {
+        hipStreamSynchronize(s->solo->stream);
+        s->solo->sims.clear();
+        batch_destroy_internal(s->solo);
+    }
+    hipDeviceSynchronize();
+    for (void* p : s->allocs) hipFree(p);
+    delete s;
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_set_rule(onc_sim* s, onc_module* m, const char* name) {
+    if (!s) return fail(ONC_ERR_INVALID, "NULL sim");
+    if (!m) {
+        s->rule_fn = nullptr;
+        s->rule_mod = nullptr;
+    } else {
+        if (!name) return fail(ONC_ERR_INVALID, "kernel name is NULL");
+        CK(hipSetDevice(s->ctx->device));
+        hipFunction_t f = nullptr;
+        hipError_t e = hipModuleGetFunction(&f, m->mod, name);
+        if (e != hipSuccess) return fail(ONC_ERR_INVALID, std::string("no kernel named ") + name + " in module");
+        s->rule_fn = f;
+        s->rule_mod = m;
+    }
+    if (s->owner) s->owner->dirty = true;
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_set_params(onc_sim* s, const float* params, int n) {
+    if (!s || (n > 0 && !params)) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (n < 0 || n > ONC_MAX_PARAMS) return fail(ONC_ERR_INVALID, "n must be in [0, 32]");
+    for (int i = 0; i < n; i++) s->desc.params[i] = params[i];
+    s->desc.n_params = std::max(s->desc.n_params, n);
+    memcpy(s->dsim.params, s->desc.params, sizeof(s->dsim.params));
+    if (s->owner) s->owner->dirty = true;
+    return ONC_OK;
+}
+
+static int sim_read_state(onc_sim* s, onc_dstate* st) {
+    CKR(batch_commit(s->owner));
+    CK(hipMemcpy(st, s->dstate, sizeof(*st), hipMemcpyDeviceToHost));
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_upload_cells(onc_sim* s, const void* const* cols, uint32_t n, uint32_t next_id) {
+    if (!s || (n > 0 && !cols)) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (n > s->desc.capacity) return fail(ONC_ERR_CAPACITY, "more cells than capacity");
+    CK(hipSetDevice(s->ctx->device));
+    onc_dstate st;
+    CKR(sim_read_state(s, &st));
+    for (int c = 0; c < s->desc.n_cols && n > 0; c++) {
+        if (!cols[c]) return fail(ONC_ERR_INVALID, "column pointer is NULL");
+        CK(hipMemcpy(s->dsim.st[st.parity][c], cols[c], (size_t)n * 4, hipMemcpyHostToDevice));
+    }
+    st.n = n;
+    st.next_id = next_id;
+    CK(hipMemcpy(s->dstate, &st, sizeof(st), hipMemcpyHostToDevice));
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_download_cells(onc_sim* s, void* const* cols, uint32_t cap, uint32_t* n) {
+    if (!s) return fail(ONC_ERR_INVALID, "NULL sim");
+    CK(hipSetDevice(s->ctx->device));
+    onc_dstate st;
+    CKR(sim_read_state(s, &st));
+    if (n) *n = st.n;
+    uint32_t m = std::min(cap, st.n);
+    if (cols && m > 0)
+        for (int c = 0; c < s->desc.n_cols; c++)
+            if (cols[c]) CK(hipMemcpy(cols[c], s->dsim.st[st.parity][c], (size_t)m * 4, hipMemcpyDeviceToHost));
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_upload_field(onc_sim* s, int f, const float* host) {
+    if (!s || !host) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (f < 0 || f >= s->desc.n_fields) return fail(ONC_ERR_INVALID, "no such field");
+    CK(hipSetDevice(s->ctx->device));
+    onc_dstate st;
+    CKR(sim_read_state(s, &st));
+    CK(hipMemcpy(s->dsim.fld[st.parity][f], host, s->nvox * 4, hipMemcpyHostToDevice));
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_download_field(onc_sim* s, int f, float* host) {
+    if (!s || !host) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (f < 0 || f >= s->desc.n_fields) return fail(ONC_ERR_INVALID, "no such field");
+    CK(hipSetDevice(s->ctx->device));
+    onc_dstate st;
+    CKR(sim_read_state(s, &st));
+    CK(hipMemcpy(host, s->dsim.fld[st.parity][f], s->nvox * 4, hipMemcpyDeviceToHost));
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_counts(onc_sim* s, uint32_t* n, uint32_t* next_id, uint32_t* steps) {
+    if (!s) return fail(ONC_ERR_INVALID, "NULL sim");
+    CK(hipSetDevice(s->ctx->device));
+    onc_dstate st;
+    CKR(sim_read_state(s, &st));
+    if (n) *n = st.n;
+    if (next_id) *next_id = st.next_id;
+    if (steps) *steps = st.step;
+    return ONC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+static int batch_create_internal(onc_ctx* c, bool solo, onc_batch** out) {
+    auto* b = new onc_batch();
+    b->ctx = c;
+    b->is_solo = solo;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(ONC_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return ONC_OK;
+}
+
+static void batch_free_tables(onc_batch* b) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    hipFree(b->d_sims);
+    hipFree(b->d_small);
+    hipFree(b->d_hist);
+    hipFree(b->d_sort_status);
+    hipFree(b->d_comp_status);
+    hipFree(b->d_jobs);
+    b->d_sims = nullptr;
+    b->d_small = nullptr;
+    b->d_hist = nullptr;
+    b->d_sort_status = nullptr;
+    b->d_comp_status = nullptr;
+    b->d_jobs = nullptr;
+    b->cap_sims = b->cap_tiles = b->cap_jobs = 0;
+}
+
+static int batch_destroy_internal(onc_batch* b) {
+    if (!b) return ONC_OK;
+    hipStreamSynchronize(b->stream);
+    batch_free_tables(b);
+    for (hipEvent_t e : b->ev) hipEventDestroy(e);
+    hipStreamDestroy(b->stream);
+    delete b;
+    return ONC_OK;
+}
+
+// (Re)build the device segment table, plan arrays and stencil job table.
+static int batch_rebuild(onc_batch* b) {
+    CK(hipSetDevice(b->ctx->device));
+    CK(hipStreamSynchronize(b->stream));
+    const size_t S = b->sims.size();
+    const size_t tile = (size_t)256 * b->items;
+    size_t max_tiles = 0;
+    int passes = 1;
+    uint64_t total_cap = 0;
+    std::vector<onc_dsim> host(S);
+    std::vector<onc_u32> jobs, jprefix;
+    onc_u32 blocks = 0;
+    for (size_t i = 0; i < S; i++) {
+        onc_sim* s = b->sims[i];
+        host[i] = s->dsim;
+        max_tiles += (s->desc.capacity + tile - 1) / tile;
+        passes = std::max(passes, (s->key_bits + 7) / 8);
+        total_cap += s->desc.capacity;
+        const int X = s->desc.dims[0], Y = s->desc.dims[1], Z = s->desc.dims[2];
+        const onc_u32 nb = (onc_u32)(((X + b->sbx - 1) / b->sbx) * ((Y + b->sby - 1) / b->sby) *
+                                     ((Z + b->sbz - 1) / b->sbz));
+        for (int f = 0; f < s->desc.n_fields; f++) {
+            jobs.push_back((onc_u32)(i << 8) | (onc_u32)f);
+            jprefix.push_back(blocks);
+            blocks += nb;
+        }
+    }
+    if (max_tiles == 0) max_tiles = 1;
+    jprefix.push_back(blocks);
+    // (re)allocate
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    if (S > b->cap_sims || max_tiles > b->cap_tiles || jobs.size() + 1 > b->cap_jobs) {
+        batch_free_tables(b);
+        size_t cs = std::max<size_t>(S, 1), ct = max_tiles, cj = jobs.size() + 1;
+        void* p;
+        if (hipMalloc(&p, cs * sizeof(onc_dsim)) != hipSuccess) return fail(ONC_ERR_NOMEM, "batch table");
+        b->d_sims = (onc_dsim*)p;
+        if (hipMalloc(&p, (2 * (cs + 1) + ONC_NCOUNTERS + 16) * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "batch small");
+        b->d_small = (onc_u32*)p;
+        if (hipMalloc(&p, cs * 4 * 256 * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "batch hist");
+        b->d_hist = (onc_u32*)p;
+        if (hipMalloc(&p, (size_t)ONC_DEV_MAX_PASSES * ct * 256 * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "sort status");
+        b->d_sort_status = (onc_u32*)p;
+        if (hipMalloc(&p, ct * 8) != hipSuccess) return fail(ONC_ERR_NOMEM, "compaction status");
+        b->d_comp_status = (onc_u64*)p;
+        if (hipMalloc(&p, 2 * cj * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "stencil jobs");
+        b->d_jobs = (onc_u32*)p;
+        b->cap_sims = cs;
+        b->cap_tiles = ct;
+        b->cap_jobs = cj;
+        CK(hipMemsetAsync(b->d_small, 0, (2 * (cs + 1) + ONC_NCOUNTERS + 16) * 4, b->stream));
+    }
+    if (S) CK(hipMemcpyAsync(b->d_sims, host.data(), S * sizeof(onc_dsim), hipMemcpyHostToDevice, b->stream));
+    std::vector<onc_u32> jj(jobs);
+    jj.resize(b->cap_jobs, 0);
+    std::vector<onc_u32> jp(jprefix);
+    jp.resize(b->cap_jobs, blocks);
+    CK(hipMemcpyAsync(b->d_jobs, jj.data(), b->cap_jobs * 4, hipMemcpyHostToDevice, b->stream));
+    CK(hipMemcpyAsync(b->d_jobs + b->cap_jobs, jp.data(), b->cap_jobs * 4, hipMemcpyHostToDevice, b->stream));
+    CK(hipStreamSynchronize(b->stream));
+    onc_kargs& a = b->ka;
+    a.sims = b->d_sims;
+    a.nsims = (onc_u32)S;
+    a.passes = (onc_u32)passes;
+    a.tile = (onc_u32)tile;
+    a.max_tiles = (onc_u32)max_tiles;
+    a.tile_prefix = b->d_small;
+    a.cell_prefix = b->d_small + (b->cap_sims + 1);
+    a.counters = b->d_small + 2 * (b->cap_sims + 1);
+    a.err = a.counters + ONC_NCOUNTERS;
+    a.hist = b->d_hist;
+    a.sort_status = b->d_sort_status;
+    a.comp_status = b->d_comp_status;
+    a.jobs = b->d_jobs;
+    a.jobs_prefix = b->d_jobs + b->cap_jobs;
+    a.njobs = (onc_u32)jobs.size();
+    a.sbx = b->sbx;
+    a.sby = b->sby;
+    a.sbz = b->sbz;
+    b->ra.sims = b->d_sims;
+    b->ra.nsims = (onc_u32)S;
+    b->ra.passes = (onc_u32)passes;
+    b->ra.cell_prefix = a.cell_prefix;
+    b->stencil_blocks = blocks;
+    b->total_cap = total_cap;
+    b->passes = passes;
+    b->dirty = false;
+    return ONC_OK;
+}
+
+static int batch_commit(onc_batch* b) {
+    if (!b) return ONC_OK;
+    CK(hipSetDevice(b->ctx->device));
+    if (b->dirty) CKR(batch_rebuild(b));
+    if (b->sims.empty()) return ONC_OK;
+    CK(onc_launch_plan(b->ka, ONC_PLAN_COMMIT, b->stream));
+    CK(hipStreamSynchronize(b->stream));
+    return ONC_OK;
+}
+
+static int grid_cells(onc_batch* b) {
+    uint64_t need = (b->total_cap + 255) / 256;
+    uint64_t lim = (uint64_t)b->ctx->cus * 8;
+    return (int)std::max<uint64_t>(1, std::min(need, lim));
+}
+static int grid_tiles(onc_batch* b) {
+    uint64_t need = b->ka.max_tiles;
+    uint64_t lim = (uint64_t)b->ctx->cus * 4;
+    return (int)std::max<uint64_t>(1, std::min(need, lim));
+}
+
+static int launch_rule(onc_batch* b) {
+    hipFunction_t fn = b->sims.empty() ? nullptr : b->sims[0]->rule_fn;
+    if (!fn) {
+        CK(onc_launch_rule_builtin(b->ra, grid_cells(b), b->stream));
+        return ONC_OK;
+    }
+    void* args[] = {&b->ra};
+    CK(hipModuleLaunchKernel(fn, (unsigned)grid_cells(b), 1, 1, 256, 1, 1, 0, b->stream, args, nullptr));
+    return ONC_OK;
+}
+
+static void prof_mark(onc_batch* b, int slot) {
+    if (b->profile) hipEventRecord(b->ev[slot], b->stream);
+}
+
+// Enqueue one timestep for every sim of the batch.
+static int enqueue_step(onc_batch* b, bool profiling) {
+    const onc_kargs& a = b->ka;
+    hipStream_t st = b->stream;
+    const int gt = grid_tiles(b), gc = grid_cells(b);
+    if (profiling) prof_mark(b, 0);
+    CK(onc_launch_plan(a, ONC_PLAN_STEP, st));
+    if (profiling) prof_mark(b, 1);
+    CK(onc_launch_morton(a, b->items, gt, st));
+    if (profiling) prof_mark(b, 2);
+    for (int p = 0; p < b->passes; p++) CK(onc_launch_sort_pass(a, b->items, p, gt, st));
+    if (profiling) prof_mark(b, 3);
+    CK(onc_launch_permute_bins(a, gc, st));
+    if (profiling) prof_mark(b, 4);
+    CK(onc_launch_forces(a, gc, st));
+    if (profiling) prof_mark(b, 5);
+    CK(onc_launch_stencil(a, b->stencil_blocks, st));
+    if (profiling) prof_mark(b, 6);
+    CKR(launch_rule(b));
+    if (profiling) prof_mark(b, 7);
+    CK(onc_launch_compact(a, b->items, gt, st));
+    if (profiling) prof_mark(b, 8);
+    CK(onc_launch_daughters(a, gt, st));
+    if (profiling) prof_mark(b, 9);
+    return ONC_OK;
+}
+
+static int check_rules(onc_batch* b) {
+    for (size_t i = 1; i < b->sims.size(); i++)
+        if (b->sims[i]->rule_fn != b->sims[0]->rule_fn)
+            return fail(ONC_ERR_STATE, "all sims of a batch must use the same rule kernel");
+    for (onc_sim* s : b->sims) {
+        if (s->rule_fn) continue;
+        const onc_sim_desc& d = s->desc;
+        // the built-in tumour rule reads age (f32) / generation (u32) columns,
+        // field 0 (O2) and five parameters
+        if (d.n_cols < 6 || d.col_types[ONC_COL_AGE] != ONC_COL_F32 || d.col_types[ONC_COL_GEN] != ONC_COL_U32 ||
+            d.n_fields < 1 || d.n_params < 5)
+            return fail(ONC_ERR_STATE,
+                        "built-in tumour rule needs columns age (f32) and generation (u32), one field and "
+                        "five parameters; set a rule module for other schemas");
+    }
+    return ONC_OK;
+}
+
+static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
+    if (nsteps < 0) return fail(ONC_ERR_INVALID, "nsteps < 0");
+    CK(hipSetDevice(b->ctx->device));
+    if (b->dirty) CKR(batch_rebuild(b));
+    if (b->sims.empty() || nsteps == 0) return ONC_OK;
+    CKR(check_rules(b));
+    if (b->profile && !use_graph) {
+        for (int k = 0; k < nsteps; k++) {
+            CKR(enqueue_step(b, true));
+            CK(hipEventSynchronize(b->ev[9]));
+            for (int ph = 0; ph < 9; ph++) {
+                float ms = 0.0f;
+                CK(hipEventElapsedTime(&ms, b->ev[ph], b->ev[ph + 1]));
+                b->phase_us[ph] += 1000.0 * ms;
+                b->phase_n[ph] += (ph == ONC_PH_SORT) ? (uint64_t)b->passes : 1u;
+            }
+        }
+        return ONC_OK;
+    }
+    if (!use_graph) {
+        for (int k = 0; k < nsteps; k++) CKR(enqueue_step(b, false));
+        return ONC_OK;
+    }
+    if (!b->exec) {
+        CK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(b, false);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc != ONC_OK) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(ONC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        b->graph = g;
+        CK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    for (int k = 0; k < nsteps; k++) CK(hipGraphLaunch(b->exec, b->stream));
+    return ONC_OK;
+}
+
+static int batch_sync_impl(onc_batch* b) {
+    CK(hipSetDevice(b->ctx->device));
+    CK(hipStreamSynchronize(b->stream));
+    if (b->dirty) return ONC_OK;
+    onc_u32 err = 0;
+    CK(hipMemcpy(&err, b->ka.err, 4, hipMemcpyDeviceToHost));
+    if (err) return fail(ONC_ERR_DEVICE, "a kernel's bounded look-back wait expired (code " + std::to_string(err) + ")");
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_step(onc_sim* s, int nsteps) {
+    if (!s) return fail(ONC_ERR_INVALID, "NULL sim");
+    if (s->owner != s->solo) return fail(ONC_ERR_STATE, "sim belongs to a batch; step the batch");
+    return batch_step_impl(s->solo, nsteps, 0);
+}
+
+extern "C" int onc_sim_sync(onc_sim* s) {
+    if (!s) return fail(ONC_ERR_INVALID, "NULL sim");
+    return batch_sync_impl(s->owner);
+}
+
+extern "C" int onc_batch_create(onc_ctx* c, onc_batch** out) {
+    if (!c || !out) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(c->device));
+    return batch_create_internal(c, false, out);
+}
+
+extern "C" int onc_batch_destroy(onc_batch* b) {
+    if (!b) return ONC_OK;
+    if (b->is_solo) return fail(ONC_ERR_INVALID, "cannot destroy a sim's private batch");
+    hipSetDevice(b->ctx->device);
+    // commit the last step, then hand every member back to its private batch
+    int rc = batch_commit(b);
+    for (onc_sim* s : b->sims) {
+        s->owner = s->solo;
+        s->solo->dirty = true;
+    }
+    if (rc != ONC_OK) {
+        batch_destroy_internal(b);
+        return rc;
+    }
+    batch_destroy_internal(b);
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_add(onc_batch* b, onc_sim* s) {
+    if (!b || !s) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (b->is_solo) return fail(ONC_ERR_INVALID, "not a user batch");
+    if (s->ctx != b->ctx) return fail(ONC_ERR_INVALID, "sim and batch belong to different contexts");
+    if (s->owner != s->solo) return fail(ONC_ERR_STATE, "sim already belongs to a batch");
+    if (!b->sims.empty() && b->sims[0]->rule_fn != s->rule_fn)
+        return fail(ONC_ERR_STATE, "all sims of a batch must use the same rule kernel");
+    CKR(batch_commit(s->solo));
+    CK(hipStreamSynchronize(b->stream));
+    b->sims.push_back(s);
+    s->owner = b;
+    b->dirty = true;
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_remove(onc_batch* b, onc_sim* s) {
+    if (!b || !s) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (s->owner != b) return fail(ONC_ERR_STATE, "sim is not in this batch");
+    CKR(batch_commit(b));
+    b->sims.erase(std::remove(b->sims.begin(), b->sims.end(), s), b->sims.end());
+    s->owner = s->solo;
+    s->solo->dirty = true;
+    b->dirty = true;
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_size(onc_batch* b, int* n) {
+    if (!b || !n) return fail(ONC_ERR_INVALID, "NULL argument");
+    *n = (int)b->sims.size();
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_step(onc_batch* b, int nsteps, int use_graph) {
+    if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
+    if (b->is_solo) return fail(ONC_ERR_INVALID, "not a user batch");
+    return batch_step_impl(b, nsteps, use_graph);
+}
+
+extern "C" int onc_batch_sync(onc_batch* b) {
+    if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
+    return batch_sync_impl(b);
+}
+
+extern "C" int onc_batch_set_tuning(onc_batch* b, int items, int bx, int by, int bz) {
+    if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
+    if (items != 8 && items != 12 && items != 16) return fail(ONC_ERR_INVALID, "items must be 8, 12 or 16");
+    if (bx < 1 || by < 1 || bx * by != 256 || bz < 1 || bz > 1024)
+        return fail(ONC_ERR_INVALID, "stencil block must satisfy bx*by == 256, 1 <= bz <= 1024");
+    CK(hipSetDevice(b->ctx->device));
+    CK(hipStreamSynchronize(b->stream));
+    b->items = items;
+    b->sbx = bx;
+    b->sby = by;
+    b->sbz = bz;
+    b->dirty = true;
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_profile(onc_batch* b, int enable) {
+    if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
+    CK(hipSetDevice(b->ctx->device));
+    if (enable && b->ev.empty()) {
+        b->ev.resize(10);
+        for (auto& e : b->ev) CK(hipEventCreate(&e));
+    }
+    b->profile = enable != 0;
+    for (int i = 0; i < ONC_PHASES; i++) {
+        b->phase_us[i] = 0.0;
+        b->phase_n[i] = 0;
+    }
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_phase_times(onc_batch* b, double* us, uint64_t* launches, int n) {
+    if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
+    for (int i = 0; i < n && i < ONC_PHASES; i++) {
+        if (us) us[i] = b->phase_us[i];
+        if (launches) launches[i] = b->phase_n[i];
+    }
+    return ONC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pure kernels
+// ---------------------------------------------------------------------------
+static int ctx_scratch(onc_ctx* c, size_t bytes, void** out) {
+    if (bytes > c->scratch_bytes) {
+        CK(hipStreamSynchronize(c->stream));
+        for (void* p : c->scratch) hipFree(p);
+        c->scratch.clear();
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return fail(ONC_ERR_NOMEM, "scratch allocation");
+        c->scratch.push_back(p);
+        c->scratch_bytes = bytes;
+    }
+    *out = c->scratch[0];
+    return ONC_OK;
+}
+
+extern "C" int onc_morton_keys(onc_ctx* c, const float* px, const float* py, const float* pz, uint32_t n,
+                               const int32_t dims[3], float dx, int fmt, uint32_t* keys) {
+    if (!c || !dims || (n && (!px || !py || !pz || !keys))) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (fmt != ONC_KEY_MORTON30 && fmt != ONC_KEY_MORTON32) return fail(ONC_ERR_INVALID, "bad key format");
+    CK(hipSetDevice(c->device));
+    CK(onc_launch_morton_keys(px, py, pz, n, dims[0], dims[1], dims[2], dx, fmt, keys, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return ONC_OK;
+}
+
+extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* vin, uint32_t n, int key_bits,
+                              uint32_t* kout, uint32_t* vout) {
+    if (!c || (n && (!kin || !vin || !kout || !vout))) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (key_bits < 1 || key_bits > 32) return fail(ONC_ERR_INVALID, "key_bits must be in [1, 32]");
+    if (n > (1u << 30)) return fail(ONC_ERR_INVALID, "n > 2^30");
+    CK(hipSetDevice(c->device));
+    if (n == 0) return ONC_OK;
+    const int items = 16;
+    const size_t tile = 256 * items;
+    const size_t tiles = (n + tile - 1) / tile;
+    const int passes = (key_bits + 7) / 8;
+    // layout: keys[2], vals[2], dsim, dstate, small, hist, sort status, comp status
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    size_t o_k0 = take(n * 4ull), o_k1 = take(n * 4ull), o_v0 = take(n * 4ull), o_v1 = take(n * 4ull);
+    size_t o_sim = take(sizeof(onc_dsim)), o_st = take(sizeof(onc_dstate));
+    size_t o_small = take((2 * 2 + ONC_NCOUNTERS + 16) * 4);
+    size_t o_hist = take(4 * 256 * 4);
+    size_t o_ss = take((size_t)ONC_DEV_MAX_PASSES * tiles * 256 * 4);
+    size_t o_cs = take(tiles * 8);
+    void* base;
+    CKR(ctx_scratch(c, off, &base));
+    char* B = (char*)base;
+    onc_dsim ds{};
+    ds.keys[0] = (onc_u32*)(B + o_k0);
+    ds.keys[1] = (onc_u32*)(B + o_k1);
+    ds.vals[0] = (onc_u32*)(B + o_v0);
+    ds.vals[1] = (onc_u32*)(B + o_v1);
+    ds.state = (onc_dstate*)(B + o_st);
+    ds.cap = n;
+    onc_dstate dst{};
+    dst.n = n;
+    hipStream_t st = c->stream;
+    CK(hipMemcpyAsync(B + o_sim, &ds, sizeof(ds), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(B + o_st, &dst, sizeof(dst), hipMemcpyHostToDevice, st));
+    CK(hipMemsetAsync(B + o_small, 0, (2 * 2 + ONC_NCOUNTERS + 16) * 4, st));
+    CK(hipMemcpyAsync(ds.keys[0], kin, n * 4ull, hipMemcpyDeviceToDevice, st));
+    CK(hipMemcpyAsync(ds.vals[0], vin, n * 4ull, hipMemcpyDeviceToDevice, st));
+    onc_kargs a{};
+    a.sims = (const onc_dsim*)(B + o_sim);
+    a.nsims = 1;
+    a.passes = (onc_u32)passes;
+    a.tile = (onc_u32)tile;
+    a.max_tiles = (onc_u32)tiles;
+    onc_u32* small = (onc_u32*)(B + o_small);
+    a.tile_prefix = small;
+    a.cell_prefix = small + 2;
+    a.counters = small + 4;
+    a.err = a.counters + ONC_NCOUNTERS;
+    a.hist = (onc_u32*)(B + o_hist);
+    a.sort_status = (onc_u32*)(B + o_ss);
+    a.comp_status = (onc_u64*)(B + o_cs);
+    const int grid = (int)std::min<size_t>(tiles, (size_t)c->cus * 4);
+    CK(onc_launch_plan(a, ONC_PLAN_STEP, st));
+    CK(onc_launch_hist_only(a, items, grid, st));
+    for (int p = 0; p < passes; p++) CK(onc_launch_sort_pass(a, items, p, grid, st));
+    CK(hipMemcpyAsync(kout, ds.keys[passes & 1], n * 4ull, hipMemcpyDeviceToDevice, st));
+    CK(hipMemcpyAsync(vout, ds.vals[passes & 1], n * 4ull, hipMemcpyDeviceToDevice, st));
+    onc_u32 err = 0;
+    CK(hipMemcpyAsync(&err, a.err, 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (err) return fail(ONC_ERR_DEVICE, "sort look-back wait expired");
+    return ONC_OK;
+}
+
+extern "C" int onc_exscan_u32(onc_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total) {
+    if (!c || (n && (!in || !out))) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(c->device));
+    if (n == 0) {
+        if (total) CK(hipMemsetAsync(total, 0, 4, c->stream));
+        CK(hipStreamSynchronize(c->stream));
+        return ONC_OK;
+    }
+    const size_t tile = 256 * 16;
+    const size_t tiles = (n + tile - 1) / tile;
+    void* base;
+    CKR(ctx_scratch(c, tiles * 8 + 256, &base));
+    onc_u64* status = (onc_u64*)base;
+    onc_u32* counter = (onc_u32*)((char*)base + tiles * 8);
+    onc_u32* err = counter + 1;
+    CK(hipMemsetAsync(base, 0, tiles * 8 + 256, c->stream));
+    const int grid = (int)std::min<size_t>(tiles, (size_t)c->cus * 4);
+    CK(onc_launch_exscan(in, out, n, total, (onc_u32*)status, counter, err, grid, c->stream));
+    onc_u32 herr = 0;
+    CK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    if (herr) return fail(ONC_ERR_DEVICE, "scan look-back wait expired");
+    return ONC_OK;
+}
+
+extern "C" int onc_bins(onc_ctx* c, const uint32_t* keys, uint32_t n, uint32_t* start, uint32_t* end) {
+    if (!c || (n && (!keys || !start || !end))) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(c->device));
+    CK(onc_launch_bins(keys, n, start, end, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return ONC_OK;
+}
+
+extern "C" int onc_stencil7(onc_ctx* c, const float* in, float* out, const int32_t dims[3],
+                            const onc_field_desc* f) {
+    if (!c || !in || !out || !dims || !f) return fail(ONC_ERR_INVALID, "NULL argument");
+    for (int i = 0; i < 3; i++)
+        if (dims[i] < 1 || dims[i] > 2048) return fail(ONC_ERR_INVALID, "dims out of range");
+    CK(hipSetDevice(c->device));
+    const int sbx = 64, sby = 4, sbz = 16;
+    onc_dsim ds{};
+    for (int i = 0; i < 3; i++) ds.dims[i] = dims[i];
+    memcpy(&ds.fields[0], f, sizeof(onc_dfield));
+    ds.n_fields = 1;
+    ds.fld[0][0] = (float*)in;
+    ds.fld[1][0] = out;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    size_t o_sim = take(sizeof(onc_dsim)), o_st = take(sizeof(onc_dstate)), o_j = take(16);
+    void* base;
+    CKR(ctx_scratch(c, off, &base));
+    char* B = (char*)base;
+    ds.state = (onc_dstate*)(B + o_st);
+    onc_dstate dst{};
+    const onc_u32 nb = (onc_u32)(((dims[0] + sbx - 1) / sbx) * ((dims[1] + sby - 1) / sby) *
+                                 ((dims[2] + sbz - 1) / sbz));
+    onc_u32 jobs[4] = {0u, 0u, nb, 0u};  // jobs[0] = job 0, jobs_prefix = {0, nb}
+    CK(hipMemcpyAsync(B + o_sim, &ds, sizeof(ds), hipMemcpyHostToDevice, c->stream));
+    CK(hipMemcpyAsync(B + o_st, &dst, sizeof(dst), hipMemcpyHostToDevice, c->stream));
+    CK(hipMemcpyAsync(B + o_j, jobs, sizeof(jobs), hipMemcpyHostToDevice, c->stream));
+    onc_kargs a{};
+    a.sims = (const onc_dsim*)(B + o_sim);
+    a.nsims = 1;
+    a.jobs = (onc_u32*)(B + o_j);
+    a.jobs_prefix = (onc_u32*)(B + o_j) + 1;
+    a.njobs = 1;
+    a.sbx = sbx;
+    a.sby = sby;
+    a.sbz = sbz;
+    CK(onc_launch_stencil(a, nb, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return ONC_OK;
+}
+
+extern "C" int onc_philox(onc_ctx* c, const uint32_t* ctr, uint32_t n, uint64_t key, uint32_t* out) {
+    if (!c || (n && (!ctr || !out))) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(c->device));
+    CK(onc_launch_philox(ctr, n, (onc_u32)key, (onc_u32)(key >> 32), out, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return ONC_OK;
+}

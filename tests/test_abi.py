@@ -1,0 +1,71 @@
+"""The C ABI library loads and exports every entry point include/onco.h
+declares; descriptor layouts agree between C and ctypes. No GPU calls."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from oncolytica_amd.hip import _abi
+from oncolytica_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "onco.h")
+
+
+def header_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\*|int)\s+(onc_\w+)\s*\(", txt, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert header_symbols() == _abi.declared_symbols()
+
+
+def test_library_exports_every_declared_symbol():
+    L = _abi.lib()
+    for name in header_symbols():
+        assert hasattr(L, name), name
+    assert L.onc_abi_version() == 1
+
+
+def test_struct_layouts_match_c():
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "onco.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(onc_field_desc), sizeof(onc_sim_desc), sizeof(onc_device_info),
+         offsetof(onc_sim_desc, fields), offsetof(onc_sim_desc, col_types), offsetof(onc_sim_desc, params));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "l.c")
+        open(p, "w").write(src)
+        exe = os.path.join(d, "l")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), p, "-o", exe], check=True)
+        got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [C.sizeof(_abi.FieldDesc), C.sizeof(_abi.SimDesc), C.sizeof(_abi.DeviceInfo),
+                   _abi.SimDesc.fields.offset, _abi.SimDesc.col_types.offset, _abi.SimDesc.params.offset]
+
+
+def test_footprint_and_validation_without_device():
+    d = W.large_c2().desc()
+    b = _abi.footprint(d)
+    # two state buffers + daughters (4 x 6 cols x 4 B), sort keys/vals/flags, two
+    # fields double-buffered, a 2^24-entry start/end table
+    cap = 2_000_000
+    expect = 4 * 6 * cap * 4 + 5 * cap * 4 + 2 * 2 * 256 ** 3 * 4 + 2 * 2 ** 24 * 4 + 256
+    assert b == expect
+    bad = W.plain_c1().desc()
+    bad.interaction_radius = 2.0
+    with pytest.raises(_abi.OncError) as e:
+        _abi.footprint(bad)
+    assert e.value.code == _abi.ERR_INVALID and "interaction_radius" in str(e.value)
+    bad = W.plain_c1().desc()
+    bad.dims[0] = 2000
+    with pytest.raises(_abi.OncError):
+        _abi.footprint(bad)

@@ -1,0 +1,158 @@
+"""Per-kernel parity of liboncohip.so against the CPU oracle: bit-exact for
+every integer output and for the binary32 stencil."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oncolytica_amd.hip import _abi
+from oncolytica_amd.hip._abi import BC_DIRICHLET, BC_POINT_SOURCE, FieldDesc, ptr
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _dims(d):
+    return np.ascontiguousarray(np.asarray(d, dtype=np.int32))
+
+
+def gpu_keys(ctx, p, dims, dx, fmt):
+    n = len(p)
+    bx, by, bz = (ctx.upload(np.ascontiguousarray(p[:, i], dtype=np.float32)) for i in range(3))
+    out = ctx.alloc(4 * n)
+    _abi.call("onc_morton_keys", ctx.h, bx.p, by.p, bz.p, n, _dims(dims).ctypes.data_as(_abi._i32p),
+              dx, fmt, out.p)
+    return out.download(np.uint32, n)
+
+
+@pytest.mark.parametrize("dims,dx,fmt", [((64, 64, 64), 1.0, 30), ((50, 50, 50), 2.0, 30),
+                                         ((256, 256, 256), 1.0, 32), ((2000, 1500, 700), 0.5, 32)])
+def test_morton_keys(ctx, dims, dx, fmt):
+    rng = np.random.default_rng(1)
+    n = 300_001
+    p = (rng.random((n, 3)) * np.array(dims) * dx * 1.02 - 0.01 * dx).astype(np.float32)
+    k = gpu_keys(ctx, p, dims, dx, fmt)
+    np.testing.assert_array_equal(k, O.keys(p[:, 0], p[:, 1], p[:, 2], dims, dx, fmt))
+
+
+def test_reference_sort_golden_through_kernels(ctx):
+    """framework/tests/test_gpu.py:17-61 inputs: GPU keys + stable sort give
+    the reference's buckets with ids in input order."""
+    g = np.load(os.path.join(GOLDEN, "sort_golden.npz"))
+    pos = g["pos"].astype(np.float32)
+    k = gpu_keys(ctx, pos, tuple(g["dims"]), float(g["voxel"][0]), 30)
+    np.testing.assert_array_equal(k, g["keys"])
+    ks, vs = gpu_sort(ctx, k, np.arange(len(k), dtype=np.uint32), 18)
+    order = np.argsort(g["keys"], kind="stable")
+    np.testing.assert_array_equal(vs, order)
+    np.testing.assert_array_equal(ks, g["keys"][order])
+
+
+def gpu_sort(ctx, k, v, bits):
+    n = len(k)
+    kin, vin = ctx.upload(k), ctx.upload(v)
+    ko, vo = ctx.alloc(4 * n), ctx.alloc(4 * n)
+    _abi.call("onc_sort_pairs", ctx.h, kin.p, vin.p, n, bits, ko.p, vo.p)
+    return ko.download(np.uint32, n), vo.download(np.uint32, n)
+
+
+@pytest.mark.parametrize("n,bits", [(0, 18), (1, 18), (7, 8), (4095, 18), (4096, 18), (4097, 18),
+                                    (100_003, 24), (1_000_000, 24), (1_000_000, 32), (3_000_017, 27)])
+def test_sort_pairs(ctx, n, bits):
+    print(f"sort n={n} bits={bits}", flush=True)
+    rng = np.random.default_rng(n + bits)
+    k = rng.integers(0, 2**bits, n, dtype=np.uint64).astype(np.uint32)
+    if n > 100:
+        k[n // 4: n // 2] = k[n // 4]          # a long run of equal keys
+        k[: n // 8] = np.sort(k[: n // 8])      # a presorted stretch
+    v = np.arange(n, dtype=np.uint32)[::-1].copy()
+    ks, vs = gpu_sort(ctx, k, v, bits)
+    ok, ov = O.radix_sort_pairs(k, v, bits)
+    np.testing.assert_array_equal(ks, ok)
+    np.testing.assert_array_equal(vs, ov)
+
+
+def test_sort_morton_like_keys_many_duplicates(ctx):
+    rng = np.random.default_rng(5)
+    k = np.sort(rng.integers(0, 2**18, 500_000)).astype(np.uint32)
+    k = k[rng.permutation(len(k))]
+    k[::3] = 17
+    v = np.arange(len(k), dtype=np.uint32)
+    ks, vs = gpu_sort(ctx, k, v, 18)
+    order = np.argsort(k, kind="stable")
+    np.testing.assert_array_equal(vs, order)
+
+
+@pytest.mark.parametrize("n", [0, 1, 100, 4096, 4097, 1_000_000, 16_777_216])
+def test_exscan(ctx, n):
+    print(f"exscan n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 1000, n).astype(np.uint32)
+    if n > 10:
+        a[-5:] = 0xFFFFFFF0  # wraps mod 2^32
+    bi = ctx.upload(a) if n else ctx.alloc(4)
+    bo = ctx.alloc(max(4, 4 * n))
+    tot = ctx.alloc(4)
+    _abi.call("onc_exscan_u32", ctx.h, bi.p, bo.p, n, tot.p)
+    out = bo.download(np.uint32, n)
+    ref, rt = O.exscan(a)
+    np.testing.assert_array_equal(out, ref)
+    assert int(tot.download(np.uint32, 1)[0]) == rt
+
+
+def test_bins(ctx):
+    rng = np.random.default_rng(3)
+    table = 1 << 18
+    k = np.sort(rng.integers(0, table, 300_000)).astype(np.uint32)
+    fill = np.full(table, 0xFFFFFFFF, dtype=np.uint32)
+    kb, sb, eb = ctx.upload(k), ctx.upload(fill), ctx.upload(fill)
+    _abi.call("onc_bins", ctx.h, kb.p, len(k), sb.p, eb.p)
+    s, e = O.bins(k, table)
+    np.testing.assert_array_equal(sb.download(np.uint32, table), s)
+    np.testing.assert_array_equal(eb.download(np.uint32, table), e)
+
+
+def _fd(**kw):
+    f = FieldDesc()
+    f.rate, f.decay, f.init = kw.get("rate", 0.1), kw.get("decay", 0.0), kw.get("init", 0.0)
+    f.bc, f.bc_value = kw.get("bc", BC_DIRICHLET), kw.get("bc_value", 0.0)
+    f.src_radius, f.uptake = kw.get("src_radius", 0.0), kw.get("uptake", 0.0)
+    f.lo, f.hi = kw.get("lo", 0.0), kw.get("hi", 1.0)
+    return f
+
+
+@pytest.mark.parametrize("dims,kw", [
+    ((64, 64, 64), dict(rate=0.1, bc_value=1.0)),
+    ((256, 256, 256), dict(rate=0.05, decay=0.001, bc=BC_POINT_SOURCE, bc_value=1.0, src_radius=16.0)),
+    ((37, 91, 13), dict(rate=0.15, decay=0.01, bc=BC_POINT_SOURCE, bc_value=0.7, src_radius=5.0)),
+    ((130, 66, 200), dict(rate=0.1, bc_value=0.25, lo=0.1, hi=0.9)),
+])
+def test_stencil_bit_exact(ctx, dims, kw):
+    rng = np.random.default_rng(sum(dims))
+    u = rng.random(int(np.prod(dims))).astype(np.float32)
+    f = _fd(**kw)
+    bi = ctx.upload(u)
+    bo = ctx.alloc(u.nbytes)
+    _abi.call("onc_stencil7", ctx.h, bi.p, bo.p, _dims(dims).ctypes.data_as(_abi._i32p), f)
+    got = bo.download(np.float32, u.size)
+    ref = O.stencil7(u, dims, f, "f32")
+    np.testing.assert_array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_philox_known_answers_and_random(ctx):
+    d = json.load(open(os.path.join(GOLDEN, "philox_kat.json")))
+    for src in ("random123", "rocrand"):
+        for ctr, key, out in d[src]:
+            c = ctx.upload(np.array(ctr, dtype=np.uint32))
+            o = ctx.alloc(16)
+            _abi.call("onc_philox", ctx.h, c.p, 1, key[0] | (key[1] << 32), o.p)
+            assert [int(v) for v in o.download(np.uint32, 4)] == out
+    rng = np.random.default_rng(9)
+    ctr = rng.integers(0, 2**32, (100_000, 4), dtype=np.uint64).astype(np.uint32)
+    c = ctx.upload(ctr)
+    o = ctx.alloc(ctr.nbytes)
+    _abi.call("onc_philox", ctx.h, c.p, len(ctr), 0x123456789ABCDEF, o.p)
+    np.testing.assert_array_equal(o.download(np.uint32, ctr.size).reshape(-1, 4),
+                                  O.philox(ctr, 0x123456789ABCDEF))

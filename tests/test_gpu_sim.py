@@ -1,0 +1,139 @@
+"""Whole-timestep parity of the HIP path against the oracle on the same seeded
+inputs: every cell column and every field value bit-identical (binary32),
+plus determinism, batch == alone, and graph replay == eager launches."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oncolytica_amd import workloads as W
+from oncolytica_amd.hip._sim import Batch, Sim
+
+pytestmark = pytest.mark.gpu
+
+NAMES = W.TUMOR_COL_NAMES
+
+
+def compare(step, gcols, gfields, st):
+    ocols = st.columns()
+    assert len(gcols[0]) == len(ocols[0]), f"step {step}: live cells {len(gcols[0])} vs oracle {len(ocols[0])}"
+    for name, g, o in zip(NAMES, gcols, ocols):
+        gv, ov = g.view(np.uint32), o.view(np.uint32)
+        bad = np.nonzero(gv != ov)[0]
+        assert bad.size == 0, f"step {step}: column {name} differs at {bad[:5]} ({g[bad[:5]]} vs {o[bad[:5]]})"
+    for f, (g, o) in enumerate(zip(gfields, st.fields)):
+        bad = np.nonzero(g.view(np.uint32) != o.view(np.uint32))[0]
+        assert bad.size == 0, f"step {step}: field {f} differs at {bad[:5]} ({g[bad[:5]]} vs {o[bad[:5]]})"
+
+
+def run_pair(ctx, w, n_cells, steps, check_every=1, capacity=None):
+    d = w.desc()
+    if capacity:
+        d.capacity = capacity
+    cols = W.initial_cells(w, n_cells)
+    sim = Sim(ctx, d)
+    sim.upload_cells(cols)
+    st = O.TumorState(d, cols)
+    done = 0
+    try:
+        while done < steps:
+            k = min(check_every, steps - done)
+            sim.step(k)
+            sim.sync()
+            st.run(k)
+            done += k
+            print(f"{w.name}: step {done} n={st.n.value}", flush=True)
+            compare(done, sim.download_cells(), [sim.download_field(f) for f in range(d.n_fields)], st)
+    finally:
+        sim.close()
+    return st
+
+
+def test_plain_4k_64_first_steps_bit_exact(ctx):
+    run_pair(ctx, W.plain_c1(), None, 5, check_every=1)
+
+
+def test_plain_4k_64_100_steps_bit_exact(ctx):
+    st = run_pair(ctx, W.plain_c1(), None, 100, check_every=25)
+    assert st.next_id.value > 4096  # divisions happened
+    assert st.n.value > 0
+
+
+def test_large_1m_256_bit_exact(ctx):
+    w = W.large_c2()
+    run_pair(ctx, w, None, 3, check_every=1)
+
+
+def test_ensemble_member_bit_exact(ctx):
+    w = W.ensemble_c3(n_sims=4, scale=0.2)[1]
+    run_pair(ctx, w, None, 30, check_every=10)
+
+
+def test_empty_and_single_cell(ctx):
+    w = W.plain_c1()
+    run_pair(ctx, w, 1, 40, check_every=20)
+    d = w.desc()
+    sim = Sim(ctx, d)
+    try:
+        sim.upload_cells(W.initial_cells(w, 0))
+        sim.step(3)
+        sim.sync()
+        assert sim.counts() == (0, 0, 3)
+        st = O.TumorState(d, W.initial_cells(w, 0))
+        st.run(3)
+        np.testing.assert_array_equal(sim.download_field(0).view(np.uint32), st.fields[0].view(np.uint32))
+    finally:
+        sim.close()
+
+
+def test_capacity_clamp_matches_oracle(ctx):
+    w = W.plain_c1()
+    run_pair(ctx, w, 2000, 40, check_every=10, capacity=2600)
+
+
+def test_deterministic_repeat_and_graph(ctx):
+    w = W.plain_c1()
+    outs = []
+    for graph in (False, True, True):
+        sim = Sim(ctx, w.desc())
+        sim.upload_cells(W.initial_cells(w))
+        b = Batch(ctx)
+        b.add(sim)
+        b.step(30, graph=graph)
+        b.sync()
+        outs.append((sim.download_cells(), sim.download_field(0)))
+        b.close()
+        sim.close()
+    for cols, f in outs[1:]:
+        for a, b_ in zip(outs[0][0], cols):
+            np.testing.assert_array_equal(a.view(np.uint32), b_.view(np.uint32))
+        np.testing.assert_array_equal(outs[0][1].view(np.uint32), f.view(np.uint32))
+
+
+def test_batch_equals_alone(ctx):
+    ws = W.ensemble_c3(n_sims=5, scale=0.05)
+    alone = []
+    for w in ws:
+        sim = Sim(ctx, w.desc())
+        sim.upload_cells(W.initial_cells(w))
+        sim.step(12)
+        sim.sync()
+        alone.append((sim.download_cells(), sim.download_field(0)))
+        sim.close()
+    sims = []
+    b = Batch(ctx)
+    for w in ws:
+        s = Sim(ctx, w.desc())
+        s.upload_cells(W.initial_cells(w))
+        b.add(s)
+        sims.append(s)
+    b.step(5, graph=True)
+    b.step(7, graph=True)
+    b.sync()
+    for s, (cols, f) in zip(sims, alone):
+        got = s.download_cells()
+        for a, g in zip(cols, got):
+            np.testing.assert_array_equal(a.view(np.uint32), g.view(np.uint32))
+        np.testing.assert_array_equal(f.view(np.uint32), s.download_field(0).view(np.uint32))
+    b.close()
+    for s in sims:
+        s.close()
