@@ -64,8 +64,11 @@ struct onc_dsim {
     onc_u32* keys[2];
     onc_u32* vals[2];
     onc_u32* flags;
-    onc_u32* bin_start;  // indexed by Morton key; valid iff start < n and
-    onc_u32* bin_end;    //   keys_sorted[start] == key
+    // Bin table indexed by Morton key, 16 B per entry {start, end, epoch, 0}.
+    // An entry is valid in the step whose epoch (completed steps + 1) it
+    // carries; only occupied bins are written, so nothing is cleared per step.
+    onc_u32* bins;
+    float* pos4;  // sorted positions as {x, y, z, 0}, written with the permute
 };
 
 // Arguments of every batched kernel.
@@ -229,6 +232,51 @@ __device__ __forceinline__ int onc_find_seg(const onc_u32* prefix, onc_u32 nsegs
     return lo;
 }
 
+// Value known to be equal on every lane of the wave: lets the compiler keep
+// it (and what is loaded through it) in scalar registers.
+__device__ __forceinline__ onc_u32 onc_uniform(onc_u32 x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// XCD-aware block order. The 8 XCDs of an MI355X (each with its own L2) take
+// workgroups round-robin by blockIdx; this bijection gives XCD k a contiguous
+// run of logical block ids, so blocks that share halo data or neighbour bins
+// run on the same XCD and hit the same L2.
+__device__ __forceinline__ onc_u32 onc_xcd_block(onc_u32 bid, onc_u32 nblocks) {
+    const onc_u32 q = nblocks >> 3, r = nblocks & 7u;
+    const onc_u32 xcd = bid & 7u, idx = bid >> 3;
+    return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+
+// Persistent walk over the live cells of every sim of the segment table, one
+// wave64 chunk of 64 consecutive cells at a time. Each wave takes a contiguous
+// range of chunks (waves of an XCD take adjacent ranges), so a wave's Morton
+// neighbourhood stays in one L2. When a chunk lies inside one sim (the common
+// case) the sim index is wave-uniform, so the segment-table entry and the
+// column pointers are read with scalar loads. `f(s, i)` gets the sim and the
+// cell's index in it.
+template <class F>
+__device__ __forceinline__ void onc_for_cells(const onc_u32* cell_prefix, onc_u32 nsims, F f) {
+    const onc_u32 total = cell_prefix[nsims];
+    const onc_u32 lane = threadIdx.x & 63u;
+    const onc_u32 wpb = blockDim.x >> 6;
+    const onc_u32 wave = onc_uniform(onc_xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6));
+    const onc_u32 nwaves = gridDim.x * wpb;
+    const onc_u32 nchunks = (total + 63u) >> 6;
+    const onc_u32 c0 = (onc_u32)(((unsigned long long)nchunks * wave) / nwaves);
+    const onc_u32 c1 = (onc_u32)(((unsigned long long)nchunks * (wave + 1u)) / nwaves);
+    for (onc_u32 c = c0; c < c1; c++) {
+        const onc_u32 base = c * 64u;
+        const onc_u32 g = base + lane;
+        const int s0 = (int)onc_uniform((onc_u32)onc_find_seg(cell_prefix, nsims, base));
+        const onc_u32 lo = cell_prefix[s0], hi = cell_prefix[s0 + 1];
+        if (base + 63u < hi) {
+            f(s0, g - lo);
+        } else if (g < total) {
+            const int s = onc_find_seg(cell_prefix, nsims, g);
+            f(s, g - cell_prefix[s]);
+        }
+    }
+}
+
 __device__ __forceinline__ onc_u32 onc_field_index(const onc_dsim& S, float px, float py, float pz) {
     onc_u32 X = (onc_u32)S.dims[0], Y = (onc_u32)S.dims[1];
     return (onc_u32)onc_vox1(px, S.dx, S.dims[0]) +
@@ -237,29 +285,86 @@ __device__ __forceinline__ onc_u32 onc_field_index(const onc_dsim& S, float px, 
 
 // Iterate the cells of the 27 Moore bins around (px, py, pz) in
 // template.wgsl:11-21 order (x fastest), skipping out-of-range bins and the
-// cell `self`. `body(j)` receives the sorted index of each neighbour.
+// cell `self`. `body(j)` receives the sorted index of each neighbour. The
+// nine bin entries of a z-plane are loaded together (independent 16-byte
+// loads) before any of them is walked.
 template <class F>
-__device__ __forceinline__ void onc_for_neighbors(const onc_dsim& S, const onc_u32* keys, onc_u32 n, float px,
-                                                  float py, float pz, onc_u32 self, F body) {
-    int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]), vz = onc_vox1(pz, S.dx, S.dims[2]);
+__device__ __forceinline__ void onc_for_neighbors(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
+                                                  onc_u32 self, F body) {
+    const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
+              vz = onc_vox1(pz, S.dx, S.dims[2]);
+    const uint4* bins = (const uint4*)S.bins;
     for (int oz = -1; oz <= 1; oz++) {
-        int nz = vz + oz;
+        const int nz = vz + oz;
         if (nz < 0 || nz >= S.dims[2]) continue;
-        for (int oy = -1; oy <= 1; oy++) {
-            int ny = vy + oy;
-            if (ny < 0 || ny >= S.dims[1]) continue;
-            for (int ox = -1; ox <= 1; ox++) {
-                int nx = vx + ox;
-                if (nx < 0 || nx >= S.dims[0]) continue;
-                onc_u32 key = onc_morton((onc_u32)nx, (onc_u32)ny, (onc_u32)nz, S.key_format);
-                onc_u32 s = S.bin_start[key];
-                if (s >= n || keys[s] != key) continue;
-                onc_u32 e = S.bin_end[key];
-                for (onc_u32 j = s; j < e; j++) {
-                    if (j == self) continue;
-                    body(j);
-                }
+        uint4 b[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const int ny = vy + q / 3 - 1, nx = vx + q % 3 - 1;
+            const bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
+            b[q] = make_uint4(0u, 0u, 0u, 0u);
+            if (in) b[q] = bins[onc_morton((onc_u32)nx, (onc_u32)ny, (onc_u32)nz, S.key_format)];
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            if (b[q].z != epoch) continue;
+            for (onc_u32 j = b[q].x; j < b[q].y; j++) {
+                if (j == self) continue;
+                body(j);
             }
+        }
+    }
+}
+
+// Same visiting order as onc_for_neighbors, but the candidates of a z-plane's
+// nine bins are walked as one flat list, four at a time: the four position
+// loads are issued together and the pairs are then accumulated in order.
+// `pair(o, j)` receives the neighbour's packed position {x, y, z, _} and index.
+template <class F>
+__device__ __forceinline__ void onc_for_neighbor_pos4(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
+                                                      onc_u32 self, const float4* __restrict__ pos4, F pair) {
+    const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
+              vz = onc_vox1(pz, S.dx, S.dims[2]);
+    const uint4* __restrict__ bins = (const uint4*)S.bins;
+    for (int oz = -1; oz <= 1; oz++) {
+        const int nz = vz + oz;
+        if (nz < 0 || nz >= S.dims[2]) continue;
+        onc_u32 st[9], pre[10];
+        pre[0] = 0u;
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const int ny = vy + q / 3 - 1, nx = vx + q % 3 - 1;
+            const bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
+            uint4 b = make_uint4(0u, 0u, 0u, 0u);
+            if (in) b = bins[onc_morton((onc_u32)nx, (onc_u32)ny, (onc_u32)nz, S.key_format)];
+            const onc_u32 c = b.z == epoch ? b.y - b.x : 0u;
+            st[q] = b.x;
+            pre[q + 1] = pre[q] + c;
+        }
+        const onc_u32 tot = pre[9];
+        for (onc_u32 t0 = 0; t0 < tot; t0 += 4u) {
+            onc_u32 jj[4];
+            float4 p[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const onc_u32 t = t0 + (onc_u32)u;
+                onc_u32 q = 0;
+#pragma unroll
+                for (int r = 1; r < 9; r++) q += (t >= pre[r]) ? 1u : 0u;
+                onc_u32 sq = st[0], pq = pre[0];
+#pragma unroll
+                for (int r = 1; r < 9; r++)
+                    if (q == (onc_u32)r) {
+                        sq = st[r];
+                        pq = pre[r];
+                    }
+                jj[u] = t < tot ? sq + (t - pq) : 0xFFFFFFFFu;
+                p[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (jj[u] != 0xFFFFFFFFu) p[u] = pos4[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (jj[u] != 0xFFFFFFFFu && jj[u] != self) pair(p[u], jj[u]);
         }
     }
 }

@@ -18,6 +18,9 @@
 #define ONC_FLAG_P 0x80000000u
 #define ONC_CNT_MASK 0x3FFFFFFFu
 #define ONC_SPIN_LIMIT (1u << 24)
+#define ONC_F64_A (1ull << 62)
+#define ONC_F64_P (2ull << 62)
+#define ONC_M31 0x7FFFFFFFull
 
 static_assert(sizeof(onc_dfield) == 36, "onc_dfield layout");
 
@@ -59,6 +62,85 @@ __device__ __forceinline__ onc_u32 block_exscan256(onc_u32 v, onc_u32* s_tmp, on
     __syncthreads();
     if (total) *total = tot;
     return before + x - v;
+}
+
+// Decoupled look-back for one digit of a sort tile: sums the aggregates of
+// tiles g-1, g-2, ... down to the nearest inclusive prefix, reading eight
+// status words per round so one memory round trip covers eight tiles. Tiles
+// below `first` (another sim) read as an inclusive prefix of 0.
+__device__ __forceinline__ onc_u32 lookback_digit(const onc_u32* status, onc_u32 g, onc_u32 first, onc_u32 d,
+                                                  onc_u32* err) {
+    onc_u32 prefix = 0u, j = g - 1u, spins = 0u;
+    for (;;) {
+        onc_u32 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            v[q] = (j >= first + (onc_u32)q) ? ld_relaxed(&status[(size_t)(j - (onc_u32)q) * 256u + d]) : ONC_FLAG_P;
+        int q = 0;
+        bool done = false;
+        for (; q < 8; q++) {
+            if ((v[q] & (ONC_FLAG_A | ONC_FLAG_P)) == 0u) break;
+            prefix += v[q] & ONC_CNT_MASK;
+            if (v[q] & ONC_FLAG_P) {
+                done = true;
+                break;
+            }
+        }
+        if (done) return prefix;
+        j -= (onc_u32)q;
+        if (q < 8) {
+            if (++spins > ONC_SPIN_LIMIT) {
+                atomicOr(err, 1u);
+                return prefix;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+// Wave-wide decoupled look-back over 64-bit status words (2 flag bits on top):
+// lane l reads tile j - l, so one round trip covers 64 predecessors. Returns
+// the sums of the low (bits 0..30) and high (bits 31..61) counters, or of the
+// low 32 bits when `split` is false. Must be called by all 64 lanes of a wave.
+__device__ __forceinline__ void lookback_wave64(const onc_u64* status, onc_u32 g, onc_u32 first, bool split,
+                                                onc_u32* err, onc_u32 code, onc_u32* sum_lo, onc_u32* sum_hi) {
+    const int lane = threadIdx.x & 63;
+    onc_u32 lo = 0u, hi = 0u, j = g - 1u, spins = 0u;
+    for (;;) {
+        const onc_u64 v = (j >= first + (onc_u32)lane) ? ld_relaxed64(&status[j - (onc_u32)lane]) : ONC_F64_P;
+        const onc_u64 fl = v >> 62;
+        const onc_u64 notready = __ballot(fl == 0ull);
+        const onc_u64 isP = __ballot(fl == 2ull);
+        onc_u64 win;
+        if (isP) {
+            const int p = __ffsll((unsigned long long)isP) - 1;
+            win = (p == 63) ? ~0ull : ((2ull << p) - 1ull);
+        } else {
+            win = ~0ull;
+        }
+        if (notready & win) {
+            if (++spins > ONC_SPIN_LIMIT) {
+                if (lane == 0) atomicOr(err, code);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        const bool take = (win >> lane) & 1ull;
+        onc_u32 a_ = take ? (split ? (onc_u32)(v & ONC_M31) : (onc_u32)v) : 0u;
+        onc_u32 b_ = take && split ? (onc_u32)((v >> 31) & ONC_M31) : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a_ += __shfl_xor(a_, o, 64);
+            b_ += __shfl_xor(b_, o, 64);
+        }
+        lo += a_;
+        hi += b_;
+        if (isP) break;
+        j -= 64u;
+    }
+    *sum_lo = lo;
+    *sum_hi = hi;
 }
 
 // ---------------------------------------------------------------------------
@@ -134,7 +216,7 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
     const onc_u32 total = a.tile_prefix[a.nsims];
     const onc_u32 P = a.passes;
     for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
-        const int s = onc_find_seg(a.tile_prefix, a.nsims, g);
+        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
         const onc_u32 n = S.state->n;
         const onc_u32 par = S.state->parity;
@@ -175,7 +257,7 @@ __global__ __launch_bounds__(256) void k_hist_only(onc_kargs a) {
     const onc_u32 total = a.tile_prefix[a.nsims];
     const onc_u32 P = a.passes;
     for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
-        const int s = onc_find_seg(a.tile_prefix, a.nsims, g);
+        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
         const onc_u32 n = S.state->n;
         const onc_u32 base = (g - a.tile_prefix[s]) * (onc_u32)(256 * ITEMS);
@@ -224,9 +306,9 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
         if (tid == 0) s_tile = atomicAdd(&a.counters[ONC_CTR_SORT0 + pass], 1u);
         for (int i = tid; i < 4 * 256; i += 256) s_wc[i] = 0u;
         __syncthreads();
-        const onc_u32 g = s_tile;
+        const onc_u32 g = onc_uniform(s_tile);
         if (g >= total) break;
-        const int s = onc_find_seg(a.tile_prefix, a.nsims, g);
+        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.tile_prefix[s];
         const onc_u32 n = S.state->n;
@@ -283,21 +365,7 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
             st_relaxed(mine, ONC_FLAG_P | tot);
         } else {
             st_relaxed(mine, ONC_FLAG_A | tot);
-            onc_u32 j = g - 1u, spins = 0u;
-            for (;;) {
-                onc_u32 v = ld_relaxed(&status[(size_t)j * 256u + tid]);
-                if ((v & (ONC_FLAG_A | ONC_FLAG_P)) == 0u) {
-                    if (++spins > ONC_SPIN_LIMIT) {
-                        atomicOr(a.err, 1u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                prefix += v & ONC_CNT_MASK;
-                if (v & ONC_FLAG_P) break;
-                --j;
-            }
+            prefix = lookback_digit(status, g, g - t, (onc_u32)tid, a.err);
             st_relaxed(mine, ONC_FLAG_P | (prefix + tot));
         }
         const onc_u32 hd = a.hist[((size_t)s * 4u + (onc_u32)pass) * 256u + tid];
@@ -336,21 +404,34 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
 // write bin start/end of every occupied key.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
-    const onc_u32 total = a.cell_prefix[a.nsims];
     const onc_u32 P = a.passes;
-    for (onc_u32 g = blockIdx.x * 256u + threadIdx.x; g < total; g += gridDim.x * 256u) {
-        const int s = onc_find_seg(a.cell_prefix, a.nsims, g);
+    onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = a.sims[s];
-        const onc_u32 i = g - a.cell_prefix[s];
         const onc_u32 n = S.state->n;
         const onc_u32 par = S.state->parity;
-        const onc_u32* keys = S.keys[P & 1u];
+        const onc_u32* __restrict__ keys = S.keys[P & 1u];
         const onc_u32 src = S.vals[P & 1u][i];
-        for (int c = 0; c < S.n_cols; c++) S.st[par ^ 1u][c][i] = S.st[par][c][src];
         const onc_u32 k = keys[i];
-        if (i == 0u || keys[i - 1u] != k) S.bin_start[k] = i;
-        if (i + 1u == n || keys[i + 1u] != k) S.bin_end[k] = i + 1u;
-    }
+        const onc_u32 kprev = i > 0u ? keys[i - 1u] : ~k;
+        const onc_u32 knext = i + 1u < n ? keys[i + 1u] : ~k;
+        // gather in chunks of eight columns: loads first, then stores
+        for (int c0 = 0; c0 < S.n_cols; c0 += 8) {
+            onc_u32 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = (c0 + q < S.n_cols) ? S.st[par][c0 + q][src] : 0u;
+            if (c0 == 0)
+                ((float4*)S.pos4)[i] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]),
+                                                   __uint_as_float(v[2]), 0.0f);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (c0 + q < S.n_cols) S.st[par ^ 1u][c0 + q][i] = v[q];
+        }
+        if (kprev != k) {
+            S.bins[(size_t)k * 4u] = i;
+            S.bins[(size_t)k * 4u + 2u] = S.state->step + 1u;
+        }
+        if (knext != k) S.bins[(size_t)k * 4u + 1u] = i + 1u;
+    });
 }
 
 // Fixed (Dirichlet) voxel predicate (engine/assets/shaders/tumor_sim.wgsl:420-423).
@@ -369,42 +450,64 @@ __device__ __forceinline__ bool onc_is_fixed(const onc_dfield& f, const int* d, 
 // explicit overdamped integration, and per-voxel uptake by the first cell of
 // each bin (tumor_sim.wgsl:387-412 reaction, here before the sweep).
 // ---------------------------------------------------------------------------
+#define ONC_PAIR_LIST 16
+
 __global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
-    const onc_u32 total = a.cell_prefix[a.nsims];
+    // per-thread list of interacting neighbours, [slot][thread] (bank-conflict free)
+    __shared__ onc_u32 s_pairs[ONC_PAIR_LIST][256];
     const onc_u32 P = a.passes;
-    for (onc_u32 g = blockIdx.x * 256u + threadIdx.x; g < total; g += gridDim.x * 256u) {
-        const int s = onc_find_seg(a.cell_prefix, a.nsims, g);
+    const int tid = threadIdx.x;
+    onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = a.sims[s];
-        const onc_u32 i = g - a.cell_prefix[s];
-        const onc_u32 n = S.state->n;
         const onc_u32 par = S.state->parity;
-        const onc_u32* keys = S.keys[P & 1u];
-        const float* tx = (const float*)S.st[par ^ 1u][0];
-        const float* ty = (const float*)S.st[par ^ 1u][1];
-        const float* tz = (const float*)S.st[par ^ 1u][2];
-        const float px = tx[i], py = ty[i], pz = tz[i];
+        const onc_u32 epoch = S.state->step + 1u;
+        const onc_u32* __restrict__ keys = S.keys[P & 1u];
+        const float4* __restrict__ p4 = (const float4*)S.pos4;
+        const float4 me = p4[i];
+        const float px = me.x, py = me.y, pz = me.z;
         const float R = S.R, Sc = 2.0f * S.cell_radius;
         const float R2 = R * R, RmS = R - Sc, krep = S.k_rep, kadh = S.k_adh;
         float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
-        onc_for_neighbors(S, keys, n, px, py, pz, i, [&](onc_u32 j) {
-            float ddx = px - tx[j];
-            float ddy = py - ty[j];
-            float ddz = pz - tz[j];
+        // Phase B: the pair law over the listed neighbours, in visiting order.
+        onc_u32 npair = 0;
+        auto flush = [&]() {
+            for (onc_u32 k = 0; k < npair; k++) {
+                const float4 o = p4[s_pairs[k][tid]];
+                const float ddx = px - o.x;
+                const float ddy = py - o.y;
+                const float ddz = pz - o.z;
+                float d2 = ddx * ddx + ddy * ddy;
+                d2 = d2 + ddz * ddz;
+                const float dist = __builtin_sqrtf(d2);
+                float f;
+                if (dist < Sc) {
+                    f = krep * (Sc - dist);
+                } else {
+                    if (kadh == 0.0f) continue;
+                    f = -(kadh * ((R - dist) / RmS));
+                }
+                const float sc = f / dist;
+                Fx = Fx + sc * ddx;
+                Fy = Fy + sc * ddy;
+                Fz = Fz + sc * ddz;
+            }
+            npair = 0;
+        };
+        // Phase A: cheap distance filter over the 27 bins; only pairs inside
+        // the interaction radius reach the expensive (sqrt, divide) path, so
+        // lanes diverge on short lists instead of on every candidate.
+        onc_for_neighbor_pos4(S, epoch, px, py, pz, i, p4, [&](const float4& o, onc_u32 j) {
+            const float ddx = px - o.x;
+            const float ddy = py - o.y;
+            const float ddz = pz - o.z;
             float d2 = ddx * ddx + ddy * ddy;
             d2 = d2 + ddz * ddz;
             if (!(d2 < R2) || d2 == 0.0f) return;
-            float dist = __builtin_sqrtf(d2);
-            float f;
-            if (dist < Sc) {
-                f = krep * (Sc - dist);
-            } else {
-                if (kadh == 0.0f) return;
-                f = -(kadh * ((R - dist) / RmS));
-            }
-            Fx = Fx + f * (ddx / dist);
-            Fy = Fy + f * (ddy / dist);
-            Fz = Fz + f * (ddz / dist);
+            if (npair == ONC_PAIR_LIST) flush();
+            s_pairs[npair][tid] = j;
+            npair++;
         });
+        flush();
         const float mob = S.mobility, md = S.max_disp;
         float dpx = mob * Fx, dpy = mob * Fy, dpz = mob * Fz;
         float l2 = dpx * dpx + dpy * dpy;
@@ -424,7 +527,7 @@ __global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
         // uptake, once per occupied voxel
         const onc_u32 k = keys[i];
         if (i == 0u || keys[i - 1u] != k) {
-            const onc_u32 cntb = S.bin_end[k] - i;
+            const onc_u32 cntb = S.bins[(size_t)k * 4u + 1u] - i;
             const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
                       vz = onc_vox1(pz, S.dx, S.dims[2]);
             const size_t v = (size_t)vx + (size_t)S.dims[0] * ((size_t)vy + (size_t)S.dims[1] * (size_t)vz);
@@ -436,20 +539,21 @@ __global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
                 u[v] = w < fd.lo ? fd.lo : w;
             }
         }
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
 // 7-point explicit sweep, all (sim, field) jobs in one launch. A block of
 // sbx x sby threads owns an xy tile and marches sbz planes along z, keeping
-// the z-1 / z / z+1 values of its column in registers; x/y neighbours come
-// through L1/L2. u' = clamp(u + r (sum6 - 6u) - decay u) with the oracle's
-// summation order; zero-flux faces mirror the centre.
+// the z-1 / z / z+1 values of its column in registers and loading plane z+1's
+// x/y neighbours one iteration ahead (software pipelining); x/y neighbours
+// come through L1/L2. u' = clamp(u + r (sum6 - 6u) - decay u) with the
+// oracle's summation order; zero-flux faces mirror the centre.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
-    const onc_u32 b = blockIdx.x;
-    const int jb = onc_find_seg(a.jobs_prefix, a.njobs, b);
-    const onc_u32 job = a.jobs[jb];
+    const onc_u32 b = onc_uniform(onc_xcd_block(blockIdx.x, gridDim.x));
+    const int jb = (int)onc_uniform((onc_u32)onc_find_seg(a.jobs_prefix, a.njobs, b));
+    const onc_u32 job = onc_uniform(a.jobs[jb]);
     const onc_dsim& S = a.sims[job >> 8];
     const int f = (int)(job & 255u);
     const int X = S.dims[0], Y = S.dims[1], Z = S.dims[2];
@@ -471,20 +575,32 @@ __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
     const onc_dfield fd = S.fields[f];
     const size_t XY = (size_t)X * (size_t)Y;
     const size_t col = (size_t)x + (size_t)X * (size_t)y;
-    float cc = in[col + XY * (size_t)z0];
-    float cm = z0 > 0 ? in[col + XY * (size_t)(z0 - 1)] : cc;
-    for (int z = z0; z < z1; z++) {
-        const size_t v = col + XY * (size_t)z;
-        const float cp = (z + 1 < Z) ? in[v + XY] : cc;
+    const bool hxm = x > 0, hxp = x < X - 1, hym = y > 0, hyp = y < Y - 1;
+    // plane z0: centre, below, and in-plane neighbours
+    size_t v = col + XY * (size_t)z0;
+    float cc = in[v];
+    float cm = z0 > 0 ? in[v - XY] : cc;
+    float xm = hxm ? in[v - 1] : cc;
+    float xp = hxp ? in[v + 1] : cc;
+    float ym = hym ? in[v - (size_t)X] : cc;
+    float yp = hyp ? in[v + (size_t)X] : cc;
+    for (int z = z0; z < z1; z++, v += XY) {
+        // issue plane z+1 loads before computing plane z
+        const bool nxt = z + 1 < Z;
+        const size_t vn = v + XY;
+        const float cp = nxt ? in[vn] : cc;
+        float nxm = 0.0f, nxp = 0.0f, nym = 0.0f, nyp = 0.0f;
+        if (nxt && z + 1 < z1) {
+            nxm = hxm ? in[vn - 1] : 0.0f;
+            nxp = hxp ? in[vn + 1] : 0.0f;
+            nym = hym ? in[vn - (size_t)X] : 0.0f;
+            nyp = hyp ? in[vn + (size_t)X] : 0.0f;
+        }
         if (onc_is_fixed(fd, S.dims, x, y, z)) {
             out[v] = fd.bc_value;
         } else {
-            const float xm = x > 0 ? in[v - 1] : cc;
-            const float xp = x < X - 1 ? in[v + 1] : cc;
-            const float ym = y > 0 ? in[v - (size_t)X] : cc;
-            const float yp = y < Y - 1 ? in[v + (size_t)X] : cc;
             const float zm = z > 0 ? cm : cc;
-            const float zp = z + 1 < Z ? cp : cc;
+            const float zp = nxt ? cp : cc;
             float sum = xm + xp;
             sum = sum + ym;
             sum = sum + yp;
@@ -497,6 +613,10 @@ __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
         }
         cm = cc;
         cc = cp;
+        xm = hxm ? nxm : cp;
+        xp = hxp ? nxp : cp;
+        ym = hym ? nym : cp;
+        yp = hyp ? nyp : cp;
     }
 }
 
@@ -506,11 +626,8 @@ __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
 // columns to the current buffer, its alive/divide flags and its daughter.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
-    const onc_u32 total = r.cell_prefix[r.nsims];
-    for (onc_u32 g = blockIdx.x * 256u + threadIdx.x; g < total; g += gridDim.x * 256u) {
-        const int s = onc_find_seg(r.cell_prefix, r.nsims, g);
+    onc_for_cells(r.cell_prefix, r.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = r.sims[s];
-        const onc_u32 i = g - r.cell_prefix[s];
         const onc_u32 par = S.state->parity;
         onc_u32* const* T = S.st[par ^ 1u];
         onc_u32* const* C = S.st[par];
@@ -559,7 +676,7 @@ __global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
             S.dau[5][i] = gen;
             for (int c = 6; c < S.n_cols; c++) S.dau[c][i] = T[c][i];
         }
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -567,9 +684,6 @@ __global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
 // survivors are gathered in order into the other buffer, daughters into a
 // dense staging area in parent order. The sim's last tile records totals.
 // ---------------------------------------------------------------------------
-#define ONC_F64_A (1ull << 62)
-#define ONC_F64_P (2ull << 62)
-#define ONC_M31 0x7FFFFFFFull
 
 template <int ITEMS>
 __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
@@ -584,9 +698,9 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
     for (;;) {
         if (tid == 0) s_tile = atomicAdd(&a.counters[ONC_CTR_COMPACT], 1u);
         __syncthreads();
-        const onc_u32 g = s_tile;
+        const onc_u32 g = onc_uniform(s_tile);
         if (g >= total) break;
-        const int s = onc_find_seg(a.tile_prefix, a.nsims, g);
+        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.tile_prefix[s];
         const onc_u32 ntiles_s = a.tile_prefix[s + 1] - a.tile_prefix[s];
@@ -611,37 +725,24 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
         onc_u32 tp;
         const onc_u32 ex = block_exscan256(la | (ldv << 16), s_tmp, &tp);
         const onc_u32 ta = tp & 0xFFFFu, td = tp >> 16;
-        if (tid == 0) {
-            onc_u64 pa = 0ull, pd = 0ull;
+        if (tid < 64) {
+            onc_u32 pa = 0u, pd = 0u;
             onc_u64* mine = &a.comp_status[g];
             const onc_u64 agg = (onc_u64)ta | ((onc_u64)td << 31);
             if (t == 0u) {
-                st_relaxed64(mine, ONC_F64_P | agg);
+                if (tid == 0) st_relaxed64(mine, ONC_F64_P | agg);
             } else {
-                st_relaxed64(mine, ONC_F64_A | agg);
-                onc_u32 j = g - 1u, spins = 0u;
-                for (;;) {
-                    onc_u64 v = ld_relaxed64(&a.comp_status[j]);
-                    if ((v >> 62) == 0ull) {
-                        if (++spins > ONC_SPIN_LIMIT) {
-                            atomicOr(a.err, 2u);
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    pa += v & ONC_M31;
-                    pd += (v >> 31) & ONC_M31;
-                    if ((v >> 62) == 2ull) break;
-                    --j;
-                }
-                st_relaxed64(mine, ONC_F64_P | ((pa + ta) | ((pd + td) << 31)));
+                if (tid == 0) st_relaxed64(mine, ONC_F64_A | agg);
+                lookback_wave64(a.comp_status, g, g - t, true, a.err, 2u, &pa, &pd);
+                if (tid == 0) st_relaxed64(mine, ONC_F64_P | ((onc_u64)(pa + ta) | ((onc_u64)(pd + td) << 31)));
             }
-            s_pa = (onc_u32)pa;
-            s_pd = (onc_u32)pd;
-            if (t + 1u == ntiles_s) {
-                S.state->nsurv = (onc_u32)pa + ta;
-                S.state->ndiv = (onc_u32)pd + td;
+            if (tid == 0) {
+                s_pa = pa;
+                s_pd = pd;
+                if (t + 1u == ntiles_s) {
+                    S.state->nsurv = pa + ta;
+                    S.state->ndiv = pd + td;
+                }
             }
         }
         __syncthreads();
@@ -657,19 +758,38 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
             s_dslot[tid * ITEMS + k] = (f & 2u) ? rd++ : NONE;
         }
         __syncthreads();
-        onc_u32* const* C = S.st[par];
-        onc_u32* const* T = S.st[par ^ 1u];
+        // Column-outer copy: the ITEMS loads of one column are issued together
+        // before any store (stores to one column may alias loads of another,
+        // so an item-outer loop would serialise every load behind a store).
+        onc_u32 sa[ITEMS], sd[ITEMS];
+        bool anyd = false;
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             const onc_u32 j = (onc_u32)(k * 256 + tid);
-            if (j < cnt) {
-                const onc_u32 i = base + j;
-                const onc_u32 sa = s_aslot[j];
-                if (sa != NONE)
-                    for (int c = 0; c < S.n_cols; c++) T[c][sa] = C[c][i];
-                const onc_u32 sd = s_dslot[j];
-                if (sd != NONE)
-                    for (int c = 0; c < S.n_cols; c++) S.dau2[c][sd] = S.dau[c][i];
+            sa[k] = j < cnt ? s_aslot[j] : NONE;
+            sd[k] = j < cnt ? s_dslot[j] : NONE;
+            anyd |= sd[k] != NONE;
+        }
+        for (int c = 0; c < S.n_cols; c++) {
+            const onc_u32* __restrict__ src = S.st[par][c] + base;
+            onc_u32* __restrict__ dst = S.st[par ^ 1u][c];
+            onc_u32 v[ITEMS];
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) v[k] = sa[k] != NONE ? src[k * 256 + tid] : 0u;
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++)
+                if (sa[k] != NONE) dst[sa[k]] = v[k];
+        }
+        if (anyd) {
+            for (int c = 0; c < S.n_cols; c++) {
+                const onc_u32* __restrict__ src = S.dau[c] + base;
+                onc_u32* __restrict__ dst = S.dau2[c];
+                onc_u32 v[ITEMS];
+#pragma unroll
+                for (int k = 0; k < ITEMS; k++) v[k] = sd[k] != NONE ? src[k * 256 + tid] : 0u;
+#pragma unroll
+                for (int k = 0; k < ITEMS; k++)
+                    if (sd[k] != NONE) dst[sd[k]] = v[k];
             }
         }
         __syncthreads();
@@ -681,7 +801,7 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
 __global__ __launch_bounds__(256) void k_daughters(onc_kargs a) {
     const onc_u32 total = a.tile_prefix[a.nsims];
     for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
-        const int s = onc_find_seg(a.tile_prefix, a.nsims, g);
+        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.tile_prefix[s];
         const onc_u32 nsurv = S.state->nsurv, ndiv = S.state->ndiv;
@@ -690,10 +810,12 @@ __global__ __launch_bounds__(256) void k_daughters(onc_kargs a) {
         const onc_u32 nid = S.state->next_id;
         const onc_u32 par = S.state->parity;
         onc_u32* const* T = S.st[par ^ 1u];
-        for (onc_u32 j = threadIdx.x; j < a.tile; j += 256u) {
-            const onc_u32 k = t * a.tile + j;
-            if (k < avail) {
-                for (int c = 0; c < S.n_cols; c++) T[c][nsurv + k] = (c == 3) ? nid + k : S.dau2[c][k];
+        if (t * a.tile < avail) {
+            for (onc_u32 j = threadIdx.x; j < a.tile; j += 256u) {
+                const onc_u32 k = t * a.tile + j;
+                if (k < avail) {
+                    for (int c = 0; c < S.n_cols; c++) T[c][nsurv + k] = (c == 3) ? nid + k : S.dau2[c][k];
+                }
             }
         }
         if (t == 0u && threadIdx.x == 0) {
@@ -770,7 +892,7 @@ __global__ __launch_bounds__(256) void k_exscan(const onc_u32* __restrict__ in, 
     for (;;) {
         if (tid == 0) s_tile = atomicAdd(counter, 1u);
         __syncthreads();
-        const onc_u32 g = s_tile;
+        const onc_u32 g = onc_uniform(s_tile);
         if (g >= ntiles) break;
         const onc_u32 base = g * (onc_u32)TILE;
         const onc_u32 cnt = min((onc_u32)TILE, n - base);
@@ -789,32 +911,20 @@ __global__ __launch_bounds__(256) void k_exscan(const onc_u32* __restrict__ in, 
         }
         onc_u32 tot;
         const onc_u32 ex = block_exscan256(sum, s_tmp, &tot);
-        if (tid == 0) {
-            onc_u32 pre = 0u;
+        if (tid < 64) {
+            onc_u32 pre = 0u, unused = 0u;
             onc_u64* mine = &status[g];
             if (g == 0u) {
-                st_relaxed64(mine, ONC_F64_P | (onc_u64)tot);
+                if (tid == 0) st_relaxed64(mine, ONC_F64_P | (onc_u64)tot);
             } else {
-                st_relaxed64(mine, ONC_F64_A | (onc_u64)tot);
-                onc_u32 j = g - 1u, spins = 0u;
-                for (;;) {
-                    onc_u64 w = ld_relaxed64(&status[j]);
-                    if ((w >> 62) == 0ull) {
-                        if (++spins > ONC_SPIN_LIMIT) {
-                            atomicOr(err, 4u);
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    pre += (onc_u32)w;
-                    if ((w >> 62) == 2ull) break;
-                    --j;
-                }
-                st_relaxed64(mine, ONC_F64_P | (onc_u64)(pre + tot));
+                if (tid == 0) st_relaxed64(mine, ONC_F64_A | (onc_u64)tot);
+                lookback_wave64(status, g, 0u, false, err, 4u, &pre, &unused);
+                if (tid == 0) st_relaxed64(mine, ONC_F64_P | (onc_u64)(pre + tot));
             }
-            s_pre = pre;
-            if (g + 1u == ntiles && total) *total = pre + tot;
+            if (tid == 0) {
+                s_pre = pre;
+                if (g + 1u == ntiles && total) *total = pre + tot;
+            }
         }
         __syncthreads();
         onc_u32 run = s_pre + ex;

@@ -107,6 +107,7 @@ struct onc_batch {
     // profiling
     bool profile = false;
     std::vector<hipEvent_t> ev;
+    int ev_used = 0;
     double phase_us[ONC_PHASES] = {0};
     uint64_t phase_n[ONC_PHASES] = {0};
 };
@@ -289,6 +290,9 @@ static int validate_desc(const onc_sim_desc* d) {
         int lim = i < 2 ? lim_xy : 1024;
         if (d->dims[i] < 1 || d->dims[i] > lim) return fail(ONC_ERR_INVALID, "dims out of range for key format");
     }
+    if (host_morton((uint32_t)d->dims[0] - 1, (uint32_t)d->dims[1] - 1, (uint32_t)d->dims[2] - 1, d->key_format) >=
+        0x80000000u)
+        return fail(ONC_ERR_INVALID, "grid too large: the dense bin table needs Morton keys below 2^31");
     if (!(d->dx > 0.0f)) return fail(ONC_ERR_INVALID, "dx must be > 0");
     if (d->capacity < 1 || d->capacity > (1u << 30)) return fail(ONC_ERR_INVALID, "capacity must be in [1, 2^30]");
     if (d->n_cols < 4 || d->n_cols > ONC_MAX_COLS) return fail(ONC_ERR_INVALID, "n_cols must be in [4, 48]");
@@ -321,8 +325,9 @@ static void sim_layout(const onc_sim_desc* d, uint32_t* table, int* key_bits, si
     b += 2ull * d->n_cols * cap * 4;  // state, two buffers
     b += 2ull * d->n_cols * cap * 4;  // daughters (sparse + compacted)
     b += 5ull * cap * 4;              // keys x2, vals x2, flags
+    b += 16ull * cap;                 // packed sorted positions
     b += 2ull * d->n_fields * (*nvox) * 4;
-    b += 2ull * (uint64_t)(*table) * 4;
+    b += 4ull * (uint64_t)(*table) * 4;  // bins {start, end, epoch, 0}
     b += 256;
     *bytes = b;
 }
@@ -397,6 +402,8 @@ extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) 
         s->dsim.dau[col] = (onc_u32*)p + (size_t)col * cap;
         s->dsim.dau2[col] = (onc_u32*)p + ((size_t)d->n_cols + col) * cap;
     }
+    if ((rc = sim_alloc(s, 16ull * cap, &p)) != ONC_OK) return cleanup(rc);
+    s->dsim.pos4 = (float*)p;
     if ((rc = sim_alloc(s, 5ull * cap * 4, &p)) != ONC_OK) return cleanup(rc);
     s->dsim.keys[0] = (onc_u32*)p;
     s->dsim.keys[1] = (onc_u32*)p + cap;
@@ -409,9 +416,8 @@ extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) 
             for (int f = 0; f < d->n_fields; f++)
                 s->dsim.fld[b][f] = (float*)p + ((size_t)b * d->n_fields + f) * s->nvox;
     }
-    if ((rc = sim_alloc(s, 2ull * s->table * 4, &p)) != ONC_OK) return cleanup(rc);
-    s->dsim.bin_start = (onc_u32*)p;
-    s->dsim.bin_end = (onc_u32*)p + s->table;
+    if ((rc = sim_alloc(s, 4ull * s->table * 4, &p)) != ONC_OK) return cleanup(rc);
+    s->dsim.bins = (onc_u32*)p;
     if ((rc = sim_alloc(s, sizeof(onc_dstate), &p)) != ONC_OK) return cleanup(rc);
     s->dstate = (onc_dstate*)p;
     s->dsim.state = s->dstate;
@@ -419,7 +425,7 @@ extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) 
     hipStream_t st = c->stream;
     onc_dstate z{};
     hipError_t e = hipMemcpyAsync(s->dstate, &z, sizeof(z), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = onc_launch_fill_u32(s->dsim.bin_start, 0xFFFFFFFFu, 2ull * s->table, st);
+    if (e == hipSuccess) e = onc_launch_fill_u32(s->dsim.bins, 0xFFFFFFFFu, 4ull * s->table, st);
     for (int f = 0; f < d->n_fields && e == hipSuccess; f++) {
         onc_dfield fd;
         memcpy(&fd, &d->fields[f], sizeof(fd));
@@ -721,8 +727,34 @@ static int launch_rule(onc_batch* b) {
     return ONC_OK;
 }
 
+// Profiling: 10 events per step from a pool, recorded between the phases of
+// eager (non-graph) steps; durations are accumulated at the next sync, so the
+// host never waits inside a run of steps.
 static void prof_mark(onc_batch* b, int slot) {
-    if (b->profile) hipEventRecord(b->ev[slot], b->stream);
+    hipEventRecord(b->ev[b->ev_used * 10 + slot], b->stream);
+}
+
+static int prof_reserve(onc_batch* b, int nsteps) {
+    size_t need = (size_t)(b->ev_used + nsteps) * 10;
+    while (b->ev.size() < need) {
+        hipEvent_t e;
+        CK(hipEventCreate(&e));
+        b->ev.push_back(e);
+    }
+    return ONC_OK;
+}
+
+static int prof_collect(onc_batch* b) {
+    for (int k = 0; k < b->ev_used; k++) {
+        for (int ph = 0; ph < 9; ph++) {
+            float ms = 0.0f;
+            CK(hipEventElapsedTime(&ms, b->ev[k * 10 + ph], b->ev[k * 10 + ph + 1]));
+            b->phase_us[ph] += 1000.0 * ms;
+            b->phase_n[ph] += (ph == ONC_PH_SORT) ? (uint64_t)b->passes : 1u;
+        }
+    }
+    b->ev_used = 0;
+    return ONC_OK;
 }
 
 // Enqueue one timestep for every sim of the batch.
@@ -777,15 +809,10 @@ static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
     if (b->sims.empty() || nsteps == 0) return ONC_OK;
     CKR(check_rules(b));
     if (b->profile && !use_graph) {
+        CKR(prof_reserve(b, nsteps));
         for (int k = 0; k < nsteps; k++) {
             CKR(enqueue_step(b, true));
-            CK(hipEventSynchronize(b->ev[9]));
-            for (int ph = 0; ph < 9; ph++) {
-                float ms = 0.0f;
-                CK(hipEventElapsedTime(&ms, b->ev[ph], b->ev[ph + 1]));
-                b->phase_us[ph] += 1000.0 * ms;
-                b->phase_n[ph] += (ph == ONC_PH_SORT) ? (uint64_t)b->passes : 1u;
-            }
+            b->ev_used++;
         }
         return ONC_OK;
     }
@@ -813,6 +840,7 @@ static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
 static int batch_sync_impl(onc_batch* b) {
     CK(hipSetDevice(b->ctx->device));
     CK(hipStreamSynchronize(b->stream));
+    if (b->ev_used) CKR(prof_collect(b));
     if (b->dirty) return ONC_OK;
     onc_u32 err = 0;
     CK(hipMemcpy(&err, b->ka.err, 4, hipMemcpyDeviceToHost));
@@ -916,10 +944,8 @@ extern "C" int onc_batch_set_tuning(onc_batch* b, int items, int bx, int by, int
 extern "C" int onc_batch_profile(onc_batch* b, int enable) {
     if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
     CK(hipSetDevice(b->ctx->device));
-    if (enable && b->ev.empty()) {
-        b->ev.resize(10);
-        for (auto& e : b->ev) CK(hipEventCreate(&e));
-    }
+    CK(hipStreamSynchronize(b->stream));
+    b->ev_used = 0;
     b->profile = enable != 0;
     for (int i = 0; i < ONC_PHASES; i++) {
         b->phase_us[i] = 0.0;
@@ -1115,6 +1141,23 @@ extern "C" int onc_philox(onc_ctx* c, const uint32_t* ctr, uint32_t n, uint64_t 
     if (!c || (n && (!ctr || !out))) return fail(ONC_ERR_INVALID, "NULL argument");
     CK(hipSetDevice(c->device));
     CK(onc_launch_philox(ctr, n, (onc_u32)key, (onc_u32)(key >> 32), out, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_cell_updates(onc_sim* s, uint64_t* cu) {
+    if (!s || !cu) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(s->ctx->device));
+    onc_dstate st;
+    CKR(sim_read_state(s, &st));
+    *cu = st.cell_updates;
+    return ONC_OK;
+}
+
+extern "C" int onc_copy_u32(onc_ctx* c, const uint32_t* src, uint32_t* dst, uint64_t n) {
+    if (!c || (n && (!src || !dst))) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(c->device));
+    CK(onc_launch_copy_u32(src, dst, (size_t)n, c->stream));
     CK(hipStreamSynchronize(c->stream));
     return ONC_OK;
 }

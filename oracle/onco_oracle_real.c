@@ -222,7 +222,7 @@ void FN(oracle_stencil7)(const REAL* in, REAL* out, const int32_t d[3], const on
  * taper in place of the cosine taper):
  *   d < S = 2 r_cell:      f = k_rep (S - d)            (repulsion)
  *   S <= d < R:            f = -k_adh (R - d) / (R - S) (adhesion)
- *   F_i += f * delta / d;  pairs with d >= R or d == 0 contribute nothing.
+ *   F_i += (f / d) * delta; pairs with d >= R or d == 0 contribute nothing.
  * Visitation: 27 Moore bins in template.wgsl:11-21 order (x fastest), bin
  * order inside a bin, self skipped (_stmt_translator.py:459-491).
  * Bins are found by binary search in the sorted keys here; the HIP path uses
@@ -271,9 +271,10 @@ static void force_on(const onc_sim_desc* d, const uint32_t* keys, uint32_t n, co
                         if (kadh == (REAL)0) continue;
                         f = -(kadh * ((R - dist) / RmS));
                     }
-                    Fx = Fx + f * (ddx / dist);
-                    Fy = Fy + f * (ddy / dist);
-                    Fz = Fz + f * (ddz / dist);
+                    REAL sc = f / dist; /* F += f * delta / d */
+                    Fx = Fx + sc * ddx;
+                    Fy = Fy + sc * ddy;
+                    Fz = Fz + sc * ddz;
                 }
             }
     *fx = Fx;
