@@ -59,6 +59,7 @@ extern "C" {
 #define ONC_BC_DIRICHLET 0    /* every boundary-layer voxel is held at bc_value */
 #define ONC_BC_POINT_SOURCE 1 /* zero-flux faces; boundary-layer voxels of face z=0
                                  within src_radius of the face centre are held at bc_value */
+#define ONC_BC_NONE 2         /* zero-flux faces, no fixed voxels (static or closed fields) */
 
 /* cell column element types (all columns are 32-bit) */
 #define ONC_COL_F32 0

@@ -437,7 +437,7 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // Fixed (Dirichlet) voxel predicate (engine/assets/shaders/tumor_sim.wgsl:420-423).
 __device__ __forceinline__ bool onc_is_fixed(const onc_dfield& f, const int* d, int x, int y, int z) {
     if (f.bc == 0) return x == 0 || y == 0 || z == 0 || x == d[0] - 1 || y == d[1] - 1 || z == d[2] - 1;
-    if (z != 0) return false;
+    if (f.bc != 1 || z != 0) return false;
     float fx = ((float)x + 0.5f) - (float)d[0] * 0.5f;
     float fy = ((float)y + 0.5f) - (float)d[1] * 0.5f;
     float r = f.src_radius;

@@ -306,7 +306,8 @@ static int validate_desc(const onc_sim_desc* d) {
         return fail(ONC_ERR_INVALID, "negative mechanics parameter");
     for (int f = 0; f < d->n_fields; f++) {
         const onc_field_desc& fd = d->fields[f];
-        if (fd.bc != ONC_BC_DIRICHLET && fd.bc != ONC_BC_POINT_SOURCE) return fail(ONC_ERR_INVALID, "bad field bc");
+        if (fd.bc != ONC_BC_DIRICHLET && fd.bc != ONC_BC_POINT_SOURCE && fd.bc != ONC_BC_NONE)
+            return fail(ONC_ERR_INVALID, "bad field bc");
         if (!(fd.lo <= fd.hi)) return fail(ONC_ERR_INVALID, "field lo > hi");
     }
     return ONC_OK;

@@ -32,6 +32,7 @@ KEY_MORTON30 = 30
 KEY_MORTON32 = 32
 BC_DIRICHLET = 0
 BC_POINT_SOURCE = 1
+BC_NONE = 2
 COL_F32 = 0
 COL_I32 = 1
 COL_U32 = 2

@@ -1,0 +1,266 @@
+"""HipBackend: the reference's backend plugin protocol
+(framework/src/oncolytica/core/runtime/_backend.py:6-36) on liboncohip.so.
+
+compile()      translates the cell rules to HIP C++ (hip/_translator.py),
+               builds them with hiprtc (onc_module_build), creates the device
+               sim (onc_sim_create) and uploads cells and tissue fields
+               (replaces GpuBackend.compile, gpu/runtime/_gpu_backend.py:42-104);
+run_step()     one asynchronous timestep (onc_sim_step; replaces
+               GpuBackend.run_step, _gpu_backend.py:113-171);
+sync_to_host() downloads SoA columns and fields (replaces the per-object
+               unpack of _gpu_backend.py:373-405);
+sync_to_device() uploads them (replaces _gpu_backend.py:353-371).
+
+Metric rules run on the host over the synced cells when collect_metrics is
+set (the cell objects are rebuilt as the reference's sync_to_host does).
+"""
+from __future__ import annotations
+
+import atexit
+import typing
+from typing import Any, Optional
+
+import numpy as np
+
+from .. import _types as T
+from .._errors import CompilationError
+from . import _abi
+from ._abi import FieldDesc, SimDesc
+from ._sim import Module, Sim
+from ._translator import (COL_F32, COL_I32, COL_U32, F32, CellSchema, TissueSchema, Translator,
+                          param_values)
+
+_CONTEXTS: dict[int, _abi.Context] = {}
+
+
+def context(device: int = 0) -> _abi.Context:
+    """The process-wide HIP context of a device."""
+    c = _CONTEXTS.get(device)
+    if c is None:
+        c = _abi.Context(device)
+        _CONTEXTS[device] = c
+    return c
+
+
+@atexit.register
+def _close_contexts() -> None:
+    for c in _CONTEXTS.values():
+        try:
+            c.close()
+        except Exception:
+            pass
+    _CONTEXTS.clear()
+
+
+_NP = {COL_F32: np.float32, COL_I32: np.int32, COL_U32: np.uint32}
+
+
+def _get_path(obj, path: str):
+    for p in path.split("."):
+        obj = getattr(obj, p)
+    return obj
+
+
+class HipBackend:
+    def __init__(self, engine: Any) -> None:
+        self.engine = engine
+        self.ctx = context(engine.device)
+        self.sim: Optional[Sim] = None
+        self.module: Optional[Module] = None
+        self.schema: Optional[CellSchema] = None
+        self.tissue: Optional[TissueSchema] = None
+        self.source: str = ""
+        self._next_id = 0
+
+    # ---- compile ---------------------------------------------------------------------
+    def compile(self, sim_instance: Any) -> None:
+        e = self.engine
+        if e._chemistry_rules:
+            raise CompilationError("chemistry rules are not supported by the HIP backend; declare the field on the "
+                                   "Tissue class with ol.diffusing(...) to have it diffused on the device")
+        if e._tissue_rules:
+            raise CompilationError("tissue rules are not supported by the HIP backend")
+        spec = type(sim_instance)._spec
+        self.schema = CellSchema.of(spec.cell_class)
+        self.tissue = TissueSchema.of(spec.tissue_class if e.tissue is not None else None)
+        tr = Translator(sim_instance, self.schema, self.tissue, sim_instance._params)
+        self.source = tr.translate(e._cell_rules)
+        self.module = Module(self.ctx, self.source)
+        self.sim = Sim(self.ctx, self._desc(sim_instance))
+        self.sim.set_rule(self.module, Translator.KERNEL)
+        self.sync_to_device()
+
+    def _desc(self, sim_instance) -> SimDesc:
+        e = self.engine
+        d = SimDesc()
+        d.dims[:] = list(e.tissue_shape)
+        d.dx = e.tissue_voxel_size
+        d.key_format = e.key_format
+        d.capacity = e.max_agents
+        d.seed = e.seed
+        m = e.mechanics
+        d.interaction_radius = m["interaction_radius"] if m["interaction_radius"] is not None else \
+            min(e.tissue_voxel_size, e.cell_diameter)
+        d.cell_radius = m["cell_radius"] if m["cell_radius"] is not None else 0.5 * e.cell_diameter
+        d.k_rep, d.k_adh, d.mobility, d.max_disp = m["k_rep"], m["k_adh"], m["mobility"], m["max_disp"]
+        d.n_fields = len(self.tissue.fields)
+        for f, (name, t, dflt) in enumerate(self.tissue.fields):
+            fd = FieldDesc()
+            if isinstance(dflt, T.Diffusing):
+                b = dflt.boundary
+                fd.rate, fd.decay, fd.init = dflt.rate, dflt.decay, dflt.init
+                fd.bc = _abi.BC_DIRICHLET if b.kind == "dirichlet" else _abi.BC_POINT_SOURCE
+                fd.bc_value, fd.src_radius = b.value, b.radius
+                fd.uptake, fd.lo, fd.hi = dflt.uptake, dflt.lo, dflt.hi
+            else:
+                # a static per-voxel value: swept with rate 0 (unchanged), no fixed voxels
+                fd.rate, fd.decay, fd.init = 0.0, 0.0, float(dflt or 0.0)
+                fd.bc, fd.bc_value, fd.src_radius, fd.uptake = _abi.BC_NONE, 0.0, 0.0, 0.0
+                fd.lo, fd.hi = -3.0e38, 3.0e38
+            d.fields[f] = fd
+        d.n_cols = len(self.schema.columns)
+        for k, col in enumerate(self.schema.columns):
+            d.col_types[k] = col.coltype
+        vals = param_values(sim_instance._params)
+        d.n_params = len(vals)
+        for k, v in enumerate(vals):
+            d.params[k] = v
+        return d
+
+    def update_params(self) -> None:
+        if self.sim is not None:
+            self.sim.set_params(param_values(self.engine._sim._params))
+
+    # ---- execution ---------------------------------------------------------------------
+    def run_step(self, collect_metrics: bool = False) -> None:
+        self.sim.step(1)
+        if collect_metrics and self.engine._metric_rules:
+            self.sync_to_host()
+            m = self.engine._metrics
+            m._clear()
+            for rule in self.engine._metric_rules:
+                for item in self.engine._resolve_container_for(rule):
+                    rule(item, m)
+
+    def run_steps(self, n: int) -> None:
+        self.sim.step(n)
+
+    def get_metrics(self) -> Any:
+        return self.engine._metrics
+
+    # ---- host <-> device -------------------------------------------------------------------
+    def _columns_from_objects(self, objs) -> list[np.ndarray]:
+        n = len(objs)
+        cols = []
+        for col in self.schema.columns:
+            dt = _NP[col.coltype]
+            if col.name == "_id":
+                ids = [getattr(o, "_id", None) for o in objs]
+                if any(v is None for v in ids):
+                    base = self._next_id
+                    ids = [v if v is not None else base + k for k, v in enumerate(ids)]
+                cols.append(np.asarray(ids, dtype=dt))
+                continue
+            vals = [_get_path(o, col.name) for o in objs]
+            cols.append(np.asarray(vals, dtype=np.float64).astype(dt) if n else np.zeros(0, dt))
+        return cols
+
+    def _columns_from_arrays(self, c: dict) -> list[np.ndarray]:
+        pos = np.asarray(c["pos"], dtype=np.float32)
+        n = len(pos)
+        out = []
+        for col in self.schema.columns:
+            dt = _NP[col.coltype]
+            if col.name.startswith("pos."):
+                out.append(np.ascontiguousarray(pos[:, "xyz".index(col.name[-1])]))
+            elif col.name == "_id":
+                out.append(np.asarray(c.get("_id", np.arange(n) + self._next_id), dtype=dt))
+            elif "." in col.name:
+                base, ax = col.name.split(".")
+                out.append(np.ascontiguousarray(np.asarray(c[base])[:, "xyz".index(ax)], dtype=dt))
+            elif col.name in c:
+                out.append(np.asarray(c[col.name], dtype=dt))
+            else:
+                dflt = getattr(self.schema.cls, col.name, 0)
+                out.append(np.full(n, dflt, dtype=dt))
+        return out
+
+    def sync_to_device(self) -> None:
+        e = self.engine
+        cells = e.cells
+        if cells.columns() is not None and not cells._data:
+            cols = self._columns_from_arrays(cells.columns())
+        else:
+            cells._remove_dead()
+            cols = self._columns_from_objects(cells._data)
+        n = len(cols[0])
+        if n > e.max_agents:
+            raise RuntimeError(f"{n} cells exceed max_agents={e.max_agents}")
+        next_id = max(self._next_id, int(cols[3].max()) + 1 if n else 0)
+        self.sim.upload_cells(cols, next_id=next_id)
+        self._next_id = next_id
+        if e.tissue is not None:
+            for f, (name, t, _) in enumerate(self.tissue.fields):
+                self.sim.upload_field(f, e.tissue.flat(name).astype(np.float32))
+
+    def download_columns(self) -> dict:
+        """Cells as numpy columns keyed by field name ('pos' is (n, 3))."""
+        raw = self.sim.download_cells()
+        out: dict = {"pos": np.stack(raw[:3], axis=1), "_id": raw[3]}
+        k = 4
+        for name, t in self.schema.fields:
+            if name == "pos":
+                continue
+            if t in ("vec3", "ivec3"):
+                out[name] = np.stack(raw[k:k + 3], axis=1)
+                k += 3
+            else:
+                out[name] = raw[k].astype(bool) if t == "bool" else raw[k]
+                k += 1
+        return out
+
+    def sync_to_host(self) -> None:
+        e = self.engine
+        cols = self.download_columns()
+        n_, nid, _ = self.sim.counts()
+        self._next_id = nid
+        if e.cells.columns() is not None and not e.cells._data:
+            e.cells.set_columns(cols)
+        else:
+            cls = self.schema.cls
+            objs = []
+            pos = cols["pos"]
+            others = [(name, t) for name, t in self.schema.fields if name != "pos"]
+            for i in range(len(pos)):
+                o = cls.__new__(cls)
+                object.__setattr__(o, "pos", T.vec3(*pos[i]))
+                for name, t in others:
+                    v = cols[name][i]
+                    if t == "vec3":
+                        v = T.vec3(*v)
+                    elif t == "ivec3":
+                        v = T.ivec3(*v)
+                    elif t == F32:
+                        v = float(v)
+                    elif t == "bool":
+                        v = bool(v)
+                    else:
+                        v = int(v)
+                    object.__setattr__(o, name, v)
+                o._id = int(cols["_id"][i])
+                o._alive = True
+                o._agent_list = e.cells
+                objs.append(o)
+            e.cells._data = objs
+        if e.tissue is not None:
+            for f, (name, t, _) in enumerate(self.tissue.fields):
+                arr = self.sim.download_field(f)
+                e.tissue.flat(name)[:] = arr.astype(e.tissue.flat(name).dtype)
+
+    def close(self) -> None:
+        if self.sim is not None:
+            self.sim.close()
+            self.sim = None
+        if self.module is not None:
+            self.module.close()
+            self.module = None

@@ -287,6 +287,19 @@ __device__ __forceinline__ onc_v3 onc_v3_rand_dir(onc_rng& r) { onc_v3 v; onc_ra
 '''
 
 
+def _namespace(fn) -> dict:
+    """Names a rule body can see: the module globals plus the variables it
+    closes over (models declared inside a function)."""
+    ns = dict(fn.__globals__)
+    if fn.__closure__:
+        for name, cell in zip(fn.__code__.co_freevars, fn.__closure__):
+            try:
+                ns[name] = cell.cell_contents
+            except ValueError:
+                pass
+    return ns
+
+
 @dataclass
 class Local:
     cname: str
@@ -353,7 +366,7 @@ class Translator:
             if len(args) != 2:
                 raise CompilationError(f"cell rule '{fn.__name__}' must take (self, cell)", node)
             key = f"rule:{fn.__name__}"
-            info = FuncInfo(key, f"rule_{fn.__name__}", node, fn.__globals__, "sim", VOID,
+            info = FuncInfo(key, f"rule_{fn.__name__}", node, _namespace(fn), "sim", VOID,
                             [(args[1].arg, StructT("cell", self.cell.cls))])
             self.funcs[key] = info
             self._emit_func(info, is_rule=True)
@@ -1087,7 +1100,7 @@ class Translator:
             if isinstance(ret, TupleT):
                 self.tuples.setdefault(ret.cname, ret)
             cname = "h_" + key.replace(":", "_").replace(".", "_")
-            info = FuncInfo(key, cname, fnode, fn.__globals__, recv_t, ret, params)
+            info = FuncInfo(key, cname, fnode, _namespace(fn), recv_t, ret, params)
             self.funcs[key] = info
             saved_env = getattr(self, "env", None)
             self._emit_func(info)
@@ -1186,11 +1199,8 @@ class Translator:
             out.append(self.funcs[key].code)
         calls = "\n".join(f"        {r}(cell, ctx);" for r in rule_names)
         out.append(f"""extern "C" __global__ __launch_bounds__(256) void {self.KERNEL}(onc_rule_args r) {{
-    const onc_u32 total = r.cell_prefix[r.nsims];
-    for (onc_u32 g = blockIdx.x * 256u + threadIdx.x; g < total; g += gridDim.x * 256u) {{
-        const int s = onc_find_seg(r.cell_prefix, r.nsims, g);
+    onc_for_cells(r.cell_prefix, r.nsims, [&](int s, onc_u32 i) {{
         const onc_dsim& S = r.sims[s];
-        const onc_u32 i = g - r.cell_prefix[s];
         RuleCtx ctx;
         ctx.S = &S;
         ctx.par = S.state->parity;
@@ -1207,7 +1217,7 @@ class Translator:
         {cs.cname}_store(S, S.st[ctx.par], i, cell);
         S.flags[i] = (ctx.alive ? 1u : 0u) | (ctx.div ? 2u : 0u);
         if (ctx.div) {cs.cname}_store(S, S.dau, i, ctx.daughter);
-    }}
+    }});
 }}
 """)
         return "\n".join(out)

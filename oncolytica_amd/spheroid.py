@@ -1,0 +1,63 @@
+"""The tumour-spheroid model of the benchmark workloads, written against the
+rule API: O2 consumption by resident cells, hypoxic death, drug-induced death
+with probability drug_kill * drug, division after div_age steps when
+oxygenated (daughter at daughter_offset along a random direction).
+
+Its translated rule is bit-identical to the library's built-in tumour rule
+on the same inputs (tests/test_gpu_engine.py)."""
+from __future__ import annotations
+
+import oncolytica_amd as ol
+
+
+class SpheroidTissue(ol.Tissue):
+    oxygen: ol.f32 = ol.diffusing(rate=0.1, init=1.0, boundary=ol.dirichlet(1.0), uptake=0.01)
+    drug: ol.f32 = ol.diffusing(rate=0.05, decay=0.001, init=0.0, boundary=ol.point_source(1.0, 16.0))
+
+
+class SpheroidChem(ol.Chemistry):
+    pad: ol.f32 = 0.0
+
+
+class TumorCell(ol.Cell):
+    pos: ol.vec3
+    age: ol.f32 = 0.0
+    generation: ol.u32 = 0
+
+
+class SpheroidMetrics(ol.Metrics):
+    alive: ol.u32 = 0
+
+
+class SpheroidParams(ol.Params):
+    div_age: ol.f32 = 24.0
+    div_o2: ol.f32 = 0.3
+    hypoxia: ol.f32 = 0.05
+    drug_kill: ol.f32 = 0.02
+    daughter_offset: ol.f32 = 0.5
+
+
+class Spheroid(ol.Simulation[SpheroidTissue, SpheroidChem, TumorCell, SpheroidMetrics, SpheroidParams]):
+
+    @ol.cell_rule
+    def lifecycle(self, cell: TumorCell):
+        t: SpheroidTissue = self.tissue_at(cell.pos)
+        cell.age += 1.0
+        if t.oxygen < self.params.hypoxia:
+            cell.die()
+            return
+        if self.params.drug_kill > 0.0:
+            u: ol.f32 = ol.random()
+            if u < self.params.drug_kill * t.drug:
+                cell.die()
+                return
+        if cell.age > self.params.div_age and t.oxygen > self.params.div_o2:
+            d = TumorCell(pos=cell.pos + ol.random_dir() * self.params.daughter_offset, age=0.0,
+                          generation=cell.generation + 1)
+            cell.age = 0.0
+            cell.generation += 1
+            cell.divide(d)
+
+    @ol.metric_rule
+    def count(self, cell: TumorCell, m: SpheroidMetrics):
+        m.alive += 1

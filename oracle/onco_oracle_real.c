@@ -165,7 +165,7 @@ void FN(oracle_random_dir)(uint32_t id, uint32_t step, uint64_t seed, uint32_t c
 static int is_fixed(const onc_field_desc* f, const int32_t d[3], int32_t x, int32_t y, int32_t z) {
     if (f->bc == ONC_BC_DIRICHLET)
         return x == 0 || y == 0 || z == 0 || x == d[0] - 1 || y == d[1] - 1 || z == d[2] - 1;
-    if (z != 0) return 0;
+    if (f->bc != ONC_BC_POINT_SOURCE || z != 0) return 0;
     REAL fx = ((REAL)x + (REAL)0.5) - (REAL)d[0] * (REAL)0.5;
     REAL fy = ((REAL)y + (REAL)0.5) - (REAL)d[1] * (REAL)0.5;
     REAL r = (REAL)f->src_radius;
