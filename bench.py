@@ -73,7 +73,180 @@ def cpu_baseline(w, steps: int):
                       f"oracle/onco_oracle_real.c binary32, one thread, {dt:.1f} s"}
 
 
+def _sim_cu(sim) -> int:
+    import ctypes as C
+    from oncolytica_amd.hip import _abi
+    v = C.c_uint64()
+    _abi.call("onc_sim_cell_updates", sim.h, C.byref(v))
+    return int(v.value)
+
+
+def ensemble(ctx, n_sims: int, steps: int):
+    """Batched uneven ensemble (64 sims, cells log-uniform in [10k, 500k],
+    grids from {64, 128, 192}^3, per-sim uptake / division age / hypoxia):
+    sequential plain runs (each sim alone, one after the other) against one
+    batch (one launch per phase, one hipGraph per step, autotuned tiles).
+    Both timed from resident initial state to the end of the last step; the
+    batched results are compared bit for bit with the plain ones."""
+    import numpy as np
+    from oncolytica_amd import workloads as W
+    from oncolytica_amd.hip._autotune import tune
+    from oncolytica_amd.hip._sim import Batch, Sim
+    ws = W.ensemble_c3(n_sims=n_sims)
+    cols = [W.initial_cells(w) for w in ws]
+    _log(f"ensemble: {n_sims} sims, {sum(w.n_cells for w in ws):,} cells, {steps} steps")
+    t_tune = time.perf_counter()
+    tun = tune(ctx, [(w.desc(), c) for w, c in zip(ws, cols)], steps=3)
+    t_tune = time.perf_counter() - t_tune
+    _log(f"ensemble: autotuned {tun} in {t_tune:.1f} s")
+    check = set(range(0, n_sims, max(1, n_sims // 8)))
+    plain_t, plain_cu, plain_out = 0.0, 0, {}
+    for i, (w, c) in enumerate(zip(ws, cols)):
+        s = Sim(ctx, w.desc())
+        s.upload_cells(c)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        plain_t += time.perf_counter() - t0
+        plain_cu += _sim_cu(s)
+        if i in check:
+            plain_out[i] = (s.download_cells(), s.download_field(0))
+        s.close()
+    _log(f"ensemble: sequential plain {plain_cu / plain_t / 1e6:.0f} M cell-updates/s ({plain_t:.2f} s)")
+    sims = [Sim(ctx, w.desc()) for w in ws]
+    b = Batch(ctx)
+    tun.apply(b)
+    for s, c in zip(sims, cols):
+        s.upload_cells(c)
+        b.add(s)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps, graph=True)
+    b.sync()
+    bt = time.perf_counter() - t0
+    bcu = sum(_sim_cu(s) for s in sims)
+    same = True
+    for i, (pc, pf) in plain_out.items():
+        got = sims[i].download_cells()
+        same &= all(np.array_equal(a.view(np.uint32), g.view(np.uint32)) for a, g in zip(pc, got))
+        same &= np.array_equal(pf.view(np.uint32), sims[i].download_field(0).view(np.uint32))
+    b.close()
+    for s in sims:
+        s.close()
+    _log(f"ensemble: batched {bcu / bt / 1e6:.0f} M cell-updates/s ({bt:.2f} s), "
+         f"speedup {(bcu / bt) / (plain_cu / plain_t):.2f}x, bit-identical {same}")
+    return {"workload": "ensemble_64_uneven", "sims": n_sims, "steps": steps,
+            "cells_initial": int(sum(w.n_cells for w in ws)),
+            "batched_cell_updates_per_s": bcu / bt, "batched_s": bt,
+            "plain_sequential_cell_updates_per_s": plain_cu / plain_t, "plain_sequential_s": plain_t,
+            "speedup": (bcu / bt) / (plain_cu / plain_t), "cell_updates": bcu,
+            "bit_identical_to_plain": bool(same), "sims_compared": len(plain_out),
+            "tuning": {"items": tun.items, "sbx": tun.sbx, "sby": tun.sby, "sbz": tun.sbz},
+            "autotune_s": t_tune}
+
+
+def serve(ctx, rate: float, seconds: float, budget_gb: float):
+    """Served workload: Poisson arrivals, HBM-budget admission, continuous
+    batching; per-job latency = completion - arrival (wall clock)."""
+    from oncolytica_amd.serving import Server, poisson_jobs
+    jobs = poisson_jobs(rate, seconds)
+    _log(f"serving: {len(jobs)} jobs over {seconds:.0f} s at {rate}/s, budget {budget_gb:.0f} GB")
+    srv = Server(ctx, hbm_budget=int(budget_gb * 1e9), log=None)
+    rep = srv.run(jobs)
+    out = rep.summary()
+    out.update({"workload": "served_poisson", "rate_jobs_per_s": rate, "arrival_window_s": seconds})
+    _log(f"serving: p50 {out['p50_latency_s']:.2f} s p99 {out['p99_latency_s']:.2f} s, "
+         f"{out['cell_updates_per_s'] / 1e6:.0f} M cell-updates/s, peak {out['peak_resident_jobs']} resident")
+    return out
+
+
+# rocprofv3 runs whose outputs `--summarize DIR` reduces (each under its own
+# subdirectory of DIR, written with `-d DIR/<sub> -o run`)
+PROFILE_RUNS = {
+    "kt": "rocprofv3 --kernel-trace --stats -d {d}/kt -o run -- python3 bench.py --steps 100 --warmup 5 "
+          "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0",
+    "FETCH_SIZE": "rocprofv3 --pmc FETCH_SIZE -d {d}/FETCH_SIZE -o run -- python3 bench.py --steps 20 --warmup 2 "
+                  "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0 --calibrate",
+    "WRITE_SIZE": "rocprofv3 --pmc WRITE_SIZE -d {d}/WRITE_SIZE -o run -- python3 bench.py --steps 20 --warmup 2 "
+                  "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0 --calibrate",
+}
+KERNEL_PHASE = {"k_plan": "plan", "k_morton": "morton", "k_sort_pass": "sort", "k_permute_bins": "permute_bins",
+                "k_forces": "forces", "k_stencil": "stencil", "k_rule_tumor": "rule", "k_compact": "compact",
+                "k_daughters": "daughters", "k_copy_u32": "copy"}
+COPY_BYTES = 4 * (64 << 20)  # one --calibrate copy launch reads and writes this many bytes
+
+
+def _kernel_phase(name: str):
+    base = name.split("(")[0].split()[-1].split("<")[0]
+    return KERNEL_PHASE.get(base)
+
+
+def _rocpd(path: str):
+    import glob
+    import sqlite3
+    dbs = sorted(glob.glob(os.path.join(path, "**", "*_results.db"), recursive=True))
+    if not dbs:
+        raise SystemExit(f"no rocpd database under {path}")
+    return sqlite3.connect(dbs[-1])
+
+
+def summarize(d: str, workload: str = "large_1m_256"):
+    """Reduce the PROFILE_RUNS outputs under `d` to profiles/: the kernel-trace
+    statistics, and HBM traffic per launch from FETCH_SIZE / WRITE_SIZE (KiB per
+    dispatch) scaled by what the same counters read for the plain copy kernel
+    of known size run in the same process."""
+    kt = _rocpd(os.path.join(d, "kt")).execute(
+        "select name, count(*), avg(end-start), sum(end-start), min(end-start), max(end-start) "
+        "from kernels group by name order by sum(end-start) desc").fetchall()
+    total = sum(r[3] for r in kt)
+    pmc = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        for name, v in _rocpd(os.path.join(d, ctr)).execute(
+                "select kernel_name, avg(value) from counters_collection where counter_name = ? "
+                "group by kernel_name", (ctr,)).fetchall():
+            ph = _kernel_phase(name)
+            if ph:
+                pmc.setdefault(ph, {})[ctr] = float(v)
+    cal = {c: COPY_BYTES / 1024.0 / pmc["copy"][c] for c in ("FETCH_SIZE", "WRITE_SIZE")}
+    traffic = {ph: 1024.0 * (v["FETCH_SIZE"] * cal["FETCH_SIZE"] + v["WRITE_SIZE"] * cal["WRITE_SIZE"])
+               for ph, v in pmc.items() if ph != "copy" and len(v) == 2}
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    tj = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    with open(tj, "w") as f:
+        json.dump({**{k: round(v) for k, v in traffic.items()},
+                   "_calibration": {"copy_bytes_each_way": COPY_BYTES,
+                                    "fetch_factor": round(cal["FETCH_SIZE"], 4),
+                                    "write_factor": round(cal["WRITE_SIZE"], 4)},
+                   "_commands": [PROFILE_RUNS["FETCH_SIZE"].format(d=d),
+                                 PROFILE_RUNS["WRITE_SIZE"].format(d=d)]}, f, indent=1)
+    lines = [f"# Kernel trace and HBM traffic, {workload}", "",
+             "Kernel trace (`--kernel-trace --stats`):", "",
+             "    export TMPDIR=/tmp; " + PROFILE_RUNS["kt"].format(d=d), "",
+             "| kernel | calls | avg us | min us | max us | total ms | % |", "|---|---|---|---|---|---|---|"]
+    for name, n, avg, tot, mn, mx in kt:
+        lines.append(f"| `{name.split('(')[0]}` | {n} | {avg / 1e3:.2f} | {mn / 1e3:.2f} | {mx / 1e3:.2f} | "
+                     f"{tot / 1e6:.2f} | {100.0 * tot / total:.1f} |")
+    lines += ["", "HBM traffic per launch (two `--pmc` runs, one counter each):", ""]
+    lines += ["    " + PROFILE_RUNS[c].format(d=d) for c in ("FETCH_SIZE", "WRITE_SIZE")]
+    lines += ["", f"Calibration: the plain copy kernel (`k_copy_u32`, {COPY_BYTES >> 20} MiB read + "
+                  f"{COPY_BYTES >> 20} MiB written per launch) reads FETCH_SIZE {pmc['copy']['FETCH_SIZE']:.0f} KiB "
+                  f"and WRITE_SIZE {pmc['copy']['WRITE_SIZE']:.0f} KiB, so FETCH_SIZE is scaled by "
+                  f"{cal['FETCH_SIZE']:.3f} and WRITE_SIZE by {cal['WRITE_SIZE']:.3f}.", "",
+              "| phase | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes / launch (scaled) |", "|---|---|---|---|"]
+    for ph, v in sorted(pmc.items()):
+        if ph in traffic:
+            lines.append(f"| {ph} | {v['FETCH_SIZE']:.0f} | {v['WRITE_SIZE']:.0f} | {traffic[ph]:,.0f} |")
+    md = os.path.join(ROOT, "profiles", f"kernels_{workload}.md")
+    with open(md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(f"wrote {tj} and {md}")
+
+
 def main():
+    if len(sys.argv) >= 3 and sys.argv[1] == "--summarize":
+        summarize(sys.argv[2])
+        return
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
@@ -81,6 +254,12 @@ def main():
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=5, help="oracle steps in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-sims", type=int, default=64, help="sims of the batched uneven ensemble (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="arrival window of the served workload (Poisson, 4 jobs/s; 0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--calibrate", action="store_true",
                     help="run a plain 256 MiB copy kernel 5 times first (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -202,6 +381,8 @@ def main():
             "alg_bytes_per_launch": table[dom]["alg_bytes"], "avg_launch_us": table[dom]["avg_us"]}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(w, args.cpu_steps)
+    ens = ensemble(ctx, args.ensemble_sims, args.ensemble_steps) if args.ensemble_sims > 0 else None
+    served = serve(ctx, args.serve_rate, args.serve_seconds, args.serve_budget_gb) if args.serve_seconds > 0 else None
 
     out = {
         "metric": METRIC,
@@ -225,6 +406,8 @@ def main():
         "roofline": roof,
         "phases": table,
         "cpu_baseline": cpu,
+        "ensemble": ens,
+        "serving": served,
     }
     out["graph_value"] = (gcu1 - gcu0) / (g1 - g0) if gsteps else None
     print(json.dumps(out), flush=True)

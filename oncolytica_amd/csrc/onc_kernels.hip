@@ -550,6 +550,101 @@ __global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
 // come through L1/L2. u' = clamp(u + r (sum6 - 6u) - decay u) with the
 // oracle's summation order; zero-flux faces mirror the centre.
 // ---------------------------------------------------------------------------
+// One output element of the sweep (the oracle's operation order).
+__device__ __forceinline__ float onc_sweep1(const onc_dfield& fd, float c, float xm, float xp, float ym, float yp,
+                                            float zm, float zp) {
+    float sum = xm + xp;
+    sum = sum + ym;
+    sum = sum + yp;
+    sum = sum + zm;
+    sum = sum + zp;
+    const float lap = sum - 6.0f * c;
+    float w = c + fd.rate * lap;
+    w = w - fd.decay * c;
+    return onc_clampf(w, fd.lo, fd.hi);
+}
+
+// Register-blocked sweep for rows whose length is a multiple of 4: a thread
+// owns 4 consecutive voxels of a row (float4 loads / stores) and marches sbz
+// planes along z; rows y-1, y+1 and plane z+1 come in as float4, the two x
+// halo values as scalars. A 256-thread block tiles tw = min(X/4, sbx) float4
+// columns by 256 / tw rows (sbx is a power of two <= 256).
+__device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_dfield& fd, int f, onc_u32 local,
+                                                 int sbx, int sbz) {
+    const int X = S.dims[0], Y = S.dims[1], Z = S.dims[2];
+    const int X4 = X >> 2;
+    const int tw = X4 < sbx ? X4 : sbx;
+    const int th = 256 / tw;
+    const int ntx = (X4 + tw - 1) / tw, nty = (Y + th - 1) / th;
+    const int tx = (int)(local % (onc_u32)ntx);
+    const onc_u32 rest = local / (onc_u32)ntx;
+    const int ty = (int)(rest % (onc_u32)nty);
+    const int tz = (int)(rest / (onc_u32)nty);
+    const int t = (int)threadIdx.x;
+    if (t >= tw * th) return;
+    const int x4 = tx * tw + t % tw;
+    const int y = ty * th + t / tw;
+    if (x4 >= X4 || y >= Y) return;
+    const int x0 = x4 * 4;
+    const int z0 = tz * sbz;
+    const int z1 = min(Z, z0 + sbz);
+    const onc_u32 par = S.state->parity;
+    const float* __restrict__ in = S.fld[par][f];
+    float* __restrict__ out = S.fld[par ^ 1u][f];
+    const size_t XY = (size_t)X * (size_t)Y;
+    const size_t row = (size_t)x0 + (size_t)X * (size_t)y;
+    const bool hxm = x0 > 0, hxp = x0 + 4 < X, hym = y > 0, hyp = y < Y - 1;
+    const bool dir = fd.bc == 0, psrc = fd.bc == 1;
+    size_t v = row + XY * (size_t)z0;
+    float4 c = *(const float4*)(in + v);
+    float4 cm = z0 > 0 ? *(const float4*)(in + v - XY) : c;
+    float4 ym = hym ? *(const float4*)(in + v - (size_t)X) : c;
+    float4 yp = hyp ? *(const float4*)(in + v + (size_t)X) : c;
+    float lx = hxm ? in[v - 1] : c.x;
+    float rx = hxp ? in[v + 4] : c.w;
+    for (int z = z0; z < z1; z++, v += XY) {
+        const bool nxt = z + 1 < Z;
+        const bool more = z + 1 < z1;
+        const size_t vn = v + XY;
+        const float4 cp = nxt ? *(const float4*)(in + vn) : c;
+        float4 nym = cp, nyp = cp;
+        float nlx = cp.x, nrx = cp.w;
+        if (more) {
+            if (hym) nym = *(const float4*)(in + vn - (size_t)X);
+            if (hyp) nyp = *(const float4*)(in + vn + (size_t)X);
+            if (hxm) nlx = in[vn - 1];
+            if (hxp) nrx = in[vn + 4];
+        }
+        const float4 zm = z > 0 ? cm : c;
+        const float4 zp = nxt ? cp : c;
+        float4 o;
+        o.x = onc_sweep1(fd, c.x, lx, c.y, ym.x, yp.x, zm.x, zp.x);
+        o.y = onc_sweep1(fd, c.y, c.x, c.z, ym.y, yp.y, zm.y, zp.y);
+        o.z = onc_sweep1(fd, c.z, c.y, c.w, ym.z, yp.z, zm.z, zp.z);
+        o.w = onc_sweep1(fd, c.w, c.z, rx, ym.w, yp.w, zm.w, zp.w);
+        if (dir) {
+            if (z == 0 || z == Z - 1 || y == 0 || y == Y - 1) {
+                o = make_float4(fd.bc_value, fd.bc_value, fd.bc_value, fd.bc_value);
+            } else {
+                if (x0 == 0) o.x = fd.bc_value;
+                if (x0 + 4 == X) o.w = fd.bc_value;
+            }
+        } else if (psrc && z == 0) {
+            if (onc_is_fixed(fd, S.dims, x0, y, 0)) o.x = fd.bc_value;
+            if (onc_is_fixed(fd, S.dims, x0 + 1, y, 0)) o.y = fd.bc_value;
+            if (onc_is_fixed(fd, S.dims, x0 + 2, y, 0)) o.z = fd.bc_value;
+            if (onc_is_fixed(fd, S.dims, x0 + 3, y, 0)) o.w = fd.bc_value;
+        }
+        *(float4*)(out + v) = o;
+        cm = c;
+        c = cp;
+        ym = nym;
+        yp = nyp;
+        lx = nlx;
+        rx = nrx;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
     const onc_u32 b = onc_uniform(onc_xcd_block(blockIdx.x, gridDim.x));
     const int jb = (int)onc_uniform((onc_u32)onc_find_seg(a.jobs_prefix, a.njobs, b));
@@ -559,6 +654,11 @@ __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
     const int X = S.dims[0], Y = S.dims[1], Z = S.dims[2];
     const int sbx = a.sbx, sby = a.sby, sbz = a.sbz;
     const onc_u32 local = b - a.jobs_prefix[jb];
+    if ((X & 3) == 0) {
+        const onc_dfield fdv = S.fields[f];
+        onc_stencil_vec4(S, fdv, f, local, sbx, sbz);
+        return;
+    }
     const onc_u32 nbx = (onc_u32)((X + sbx - 1) / sbx), nby = (onc_u32)((Y + sby - 1) / sby);
     const int bx = (int)(local % nbx);
     const onc_u32 rest = local / nbx;
@@ -798,31 +898,32 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
 
 // Append daughters after the survivors (capacity-clamped), assign ids, and
 // record the counters the next plan commits.
+// One thread per potential daughter: daughter k of a sim exists for k < ndiv
+// and ndiv <= n, so the walk over the step's live cells covers them all with
+// the whole chip (most threads exit at once).
 __global__ __launch_bounds__(256) void k_daughters(onc_kargs a) {
-    const onc_u32 total = a.tile_prefix[a.nsims];
-    for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
-        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
+    onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 k) {
         const onc_dsim& S = a.sims[s];
-        const onc_u32 t = g - a.tile_prefix[s];
         const onc_u32 nsurv = S.state->nsurv, ndiv = S.state->ndiv;
         const onc_u32 room = S.cap - nsurv;
         const onc_u32 avail = ndiv < room ? ndiv : room;
         const onc_u32 nid = S.state->next_id;
-        const onc_u32 par = S.state->parity;
-        onc_u32* const* T = S.st[par ^ 1u];
-        if (t * a.tile < avail) {
-            for (onc_u32 j = threadIdx.x; j < a.tile; j += 256u) {
-                const onc_u32 k = t * a.tile + j;
-                if (k < avail) {
-                    for (int c = 0; c < S.n_cols; c++) T[c][nsurv + k] = (c == 3) ? nid + k : S.dau2[c][k];
-                }
-            }
-        }
-        if (t == 0u && threadIdx.x == 0) {
+        if (k == 0u) {
             S.state->pend_n = nsurv + avail;
             S.state->pend_next_id = nid + avail;
         }
-    }
+        if (k >= avail) return;
+        const onc_u32 par = S.state->parity;
+        for (int c0 = 0; c0 < S.n_cols; c0 += 8) {
+            onc_u32 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = (c0 + q < S.n_cols) ? S.dau2[c0 + q][k] : 0u;
+            if (c0 == 0) v[3] = nid + k;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (c0 + q < S.n_cols) S.st[par ^ 1u][c0 + q][nsurv + k] = v[q];
+        }
+    });
 }
 
 // ---------------------------------------------------------------------------

@@ -100,7 +100,7 @@ struct onc_batch {
     int passes = 1;
     // tuning
     int items = 16;
-    int sbx = 64, sby = 4, sbz = 16;
+    int sbx = 64, sby = 4, sbz = 32;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -603,6 +603,19 @@ static int batch_destroy_internal(onc_batch* b) {
     return ONC_OK;
 }
 
+// Blocks of one (sim, field) sweep; must match k_stencil's decomposition:
+// rows whose length is a multiple of 4 use the float4 tiling (min(X/4, sbx)
+// float4 columns by 256 / that many rows), others sbx x sby scalar tiles;
+// both march sbz planes.
+static onc_u32 stencil_blocks(int X, int Y, int Z, int sbx, int sby, int sbz) {
+    const int nz = (Z + sbz - 1) / sbz;
+    if ((X & 3) == 0) {
+        const int X4 = X >> 2, tw = X4 < sbx ? X4 : sbx, th = 256 / tw;
+        return (onc_u32)(((X4 + tw - 1) / tw) * ((Y + th - 1) / th) * nz);
+    }
+    return (onc_u32)(((X + sbx - 1) / sbx) * ((Y + sby - 1) / sby) * nz);
+}
+
 // (Re)build the device segment table, plan arrays and stencil job table.
 static int batch_rebuild(onc_batch* b) {
     CK(hipSetDevice(b->ctx->device));
@@ -622,8 +635,7 @@ static int batch_rebuild(onc_batch* b) {
         passes = std::max(passes, (s->key_bits + 7) / 8);
         total_cap += s->desc.capacity;
         const int X = s->desc.dims[0], Y = s->desc.dims[1], Z = s->desc.dims[2];
-        const onc_u32 nb = (onc_u32)(((X + b->sbx - 1) / b->sbx) * ((Y + b->sby - 1) / b->sby) *
-                                     ((Z + b->sbz - 1) / b->sbz));
+        const onc_u32 nb = stencil_blocks(X, Y, Z, b->sbx, b->sby, b->sbz);
         for (int f = 0; f < s->desc.n_fields; f++) {
             jobs.push_back((onc_u32)(i << 8) | (onc_u32)f);
             jprefix.push_back(blocks);
@@ -780,7 +792,7 @@ static int enqueue_step(onc_batch* b, bool profiling) {
     if (profiling) prof_mark(b, 7);
     CK(onc_launch_compact(a, b->items, gt, st));
     if (profiling) prof_mark(b, 8);
-    CK(onc_launch_daughters(a, gt, st));
+    CK(onc_launch_daughters(a, gc, st));
     if (profiling) prof_mark(b, 9);
     return ONC_OK;
 }
@@ -1118,8 +1130,7 @@ extern "C" int onc_stencil7(onc_ctx* c, const float* in, float* out, const int32
     char* B = (char*)base;
     ds.state = (onc_dstate*)(B + o_st);
     onc_dstate dst{};
-    const onc_u32 nb = (onc_u32)(((dims[0] + sbx - 1) / sbx) * ((dims[1] + sby - 1) / sby) *
-                                 ((dims[2] + sbz - 1) / sbz));
+    const onc_u32 nb = stencil_blocks(dims[0], dims[1], dims[2], sbx, sby, sbz);
     onc_u32 jobs[4] = {0u, 0u, nb, 0u};  // jobs[0] = job 0, jobs_prefix = {0, nb}
     CK(hipMemcpyAsync(B + o_sim, &ds, sizeof(ds), hipMemcpyHostToDevice, c->stream));
     CK(hipMemcpyAsync(B + o_st, &dst, sizeof(dst), hipMemcpyHostToDevice, c->stream));

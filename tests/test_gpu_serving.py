@@ -1,0 +1,118 @@
+"""Batched ensemble, autotuned tile shapes and the continuous-batching server
+on the GPU: results are bit-identical whatever the tile shapes, and a served
+job's result equals the same job stepped alone."""
+import numpy as np
+import pytest
+
+from oncolytica_amd import workloads as W
+from oncolytica_amd.hip import _abi
+from oncolytica_amd.hip._autotune import Tuning, tune
+from oncolytica_amd.hip._sim import Batch, Sim
+from oncolytica_amd.serving import Job, Server, footprint, run_alone
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a_cols, a_fields, b_cols, b_fields):
+    assert len(a_cols[0]) == len(b_cols[0])
+    for a, b in zip(a_cols, b_cols):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    for a, b in zip(a_fields, b_fields):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _small_ensemble():
+    ws = W.ensemble_c3(n_sims=4, scale=0.02)
+    odd = W.plain_c1()
+    odd.dims = (50, 42, 37)   # rows not a multiple of 4: the scalar stencil path
+    odd.name = "plain_odd"
+    odd.n_cells = 1500
+    return ws + [odd]
+
+
+def _run_batch(ctx, ws, tuning, steps):
+    sims = [Sim(ctx, w.desc()) for w in ws]
+    b = Batch(ctx)
+    if tuning:
+        tuning.apply(b)
+    for s, w in zip(sims, ws):
+        s.upload_cells(W.initial_cells(w))
+        b.add(s)
+    b.step(steps // 2, graph=True)
+    b.step(steps - steps // 2, graph=False)
+    b.sync()
+    out = [(s.download_cells(), [s.download_field(f) for f in range(s.desc.n_fields)]) for s in sims]
+    b.close()
+    for s in sims:
+        s.close()
+    return out
+
+
+@pytest.mark.parametrize("tuning", [Tuning(8, 16, 16, 8), Tuning(12, 128, 2, 64), Tuning(16, 256, 1, 1)])
+def test_tile_shapes_do_not_change_results(ctx, tuning):
+    ws = _small_ensemble()
+    ref = _run_batch(ctx, ws, None, 10)
+    got = _run_batch(ctx, ws, tuning, 10)
+    for (rc, rf), (gc, gf) in zip(ref, got):
+        _same(rc, rf, gc, gf)
+
+
+def test_autotuner_returns_a_valid_shape_and_caches(ctx, tmp_path):
+    ws = W.ensemble_c3(n_sims=3, scale=0.02)
+    members = [(w.desc(), W.initial_cells(w)) for w in ws]
+    cache = str(tmp_path / "tune.json")
+    t = tune(ctx, members, steps=1, cache_file=cache)
+    assert t.items in (8, 12, 16) and t.sbx * t.sby == 256 and t.sbz >= 1
+    assert tune(ctx, members, steps=1, cache_file=cache) == t
+
+
+def test_set_tuning_rejects_bad_shapes(ctx):
+    b = Batch(ctx)
+    try:
+        for args in ((10, 64, 4, 32), (16, 64, 3, 32), (16, 64, 4, 0)):
+            with pytest.raises(_abi.OncError):
+                b.set_tuning(*args)
+    finally:
+        b.close()
+
+
+def _jobs(n, seed=7):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    jobs = []
+    t = 0.0
+    for i in range(n):
+        t += rng.exponential(0.02)
+        w, _ = W.served_job(rng, i, seed=seed, scale=0.01)
+        jobs.append(Job(id=i, workload=w, horizon=int(rng.integers(5, 60)), arrival=t))
+    return jobs
+
+
+def test_served_results_equal_alone(ctx):
+    jobs = _jobs(12)
+    rep = Server(ctx, hbm_budget=64 << 30, keep_results=True, max_tick_steps=16).run(jobs)
+    s = rep.summary()
+    assert s["completed"] == len(jobs)
+    assert rep.peak_resident > 1, "jobs should overlap on the device"
+    for j in jobs:
+        cols, fields = run_alone(ctx, j)
+        _same(cols, fields, *j.result)
+    assert rep.cell_updates == sum(j.cell_updates for j in jobs) > 0
+
+
+def test_admission_respects_hbm_budget(ctx):
+    jobs = _jobs(8, seed=3)
+    for j in jobs:
+        j.arrival = 0.0
+    fps = [footprint(j.workload.desc()) for j in jobs]
+    budget = max(fps) + min(fps)   # never room for every job at once
+    rep = Server(ctx, hbm_budget=budget, max_tick_steps=8).run(jobs)
+    assert rep.summary()["completed"] == len(jobs)
+    assert rep.peak_bytes <= budget
+    assert rep.peak_resident < len(jobs)
+    assert all(j.admitted >= j.arrival for j in jobs)
+
+
+def test_oversized_job_rejected(ctx):
+    j = _jobs(1)[0]
+    with pytest.raises(ValueError, match="budget"):
+        Server(ctx, hbm_budget=1024).run([j])
