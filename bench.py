@@ -11,9 +11,10 @@ forces + uptake, one 7-point sweep per field, the rule, compaction.
 Prints ONE JSON line with:
   value      cell-updates/s (sum of live cells at the start of each timed step
              / wall seconds), state resident in HBM, eager launches with HIP
-             events between phases (the same timed region gives `roofline`);
+             events around the dominant kernel (the kernel with the largest
+             total time in a separate pass that times every phase);
   roofline   the dominant kernel's algorithmic bytes per launch / its mean
-             event-timed duration, against 8 TB/s;
+             event-timed duration in the timed region, against 8 TB/s;
   cpu_baseline  the CPU oracle (binary32 C restatement, one thread) on a
              bounded sample of the same workload.
 """
@@ -142,7 +143,8 @@ def ensemble(ctx, n_sims: int, steps: int):
             "plain_sequential_cell_updates_per_s": plain_cu / plain_t, "plain_sequential_s": plain_t,
             "speedup": (bcu / bt) / (plain_cu / plain_t), "cell_updates": bcu,
             "bit_identical_to_plain": bool(same), "sims_compared": len(plain_out),
-            "tuning": {"items": tun.items, "sbx": tun.sbx, "sby": tun.sby, "sbz": tun.sbz},
+            "tuning": {"sort_items": tun.items, "compact_items": tun.citems, "sbx": tun.sbx, "sby": tun.sby,
+                       "sbz": tun.sbz},
             "autotune_s": t_tune}
 
 
@@ -316,10 +318,40 @@ def main():
     wb.close()
     ws.close()
 
+    # profiled pass: the same K steps with HIP events around every phase
+    # (per-phase table; events cost GPU time, so this pass is not the timed one)
+    psim, pbatch = fresh()
+    pbatch.profile(True)
+    pbatch.sync()
+    pcu0 = cell_updates(psim)
+    pbatch.step(args.steps, graph=False)
+    pbatch.sync()
+    phases = pbatch.phase_times()
+    pn_mean = (cell_updates(psim) - pcu0) / max(1, args.steps)
+    pbatch.close()
+    psim.close()
+
+    def occupied_bins(sim, n_mean):
+        import numpy as np
+        cols = sim.download_cells()
+        n = len(cols[0])
+        dims = np.array(list(desc.dims), dtype=np.int64)
+        vox = np.minimum(np.floor(np.stack(cols[:3], 1) / desc.dx).astype(np.int64), dims - 1)
+        occ_final = len(np.unique(vox[:, 0] + dims[0] * (vox[:, 1] + dims[1] * vox[:, 2])))
+        return n_mean * occ_final / max(1, n)  # occupied bins per step, scaled to the mean population
+
+    table = {}
+    for name, (us, nl) in phases.items():
+        if nl == 0:
+            continue
+        byts = algorithmic_bytes(name, pn_mean, w.nvox, len(w.fields), W.CELL_BYTES, pn_mean)
+        table[name] = {"total_ms": us / 1000.0, "launches": nl, "avg_us": us / nl}
+    dom = max((k for k in table if algorithmic_bytes(k, 1.0, 1, 1, 1, 1.0)), key=lambda k: table[k]["total_ms"])
+
     # timed region: steps 1..K of the workload from its seeded initial state,
-    # eager launches with HIP events between phases
+    # eager launches, HIP events around the dominant kernel only
     sim, batch = fresh()
-    batch.profile(True)
+    batch.profile(True, phases=[dom])
     batch.sync()
     cu0 = cell_updates(sim)
     t0 = time.perf_counter()
@@ -327,12 +359,13 @@ def main():
     batch.sync()
     t1 = time.perf_counter()
     cu1 = cell_updates(sim)
-    phases = batch.phase_times()
+    dom_us, dom_n = batch.phase_times()[dom]
     batch.profile(False)
     wall = t1 - t0
     value = (cu1 - cu0) / wall
     n_mean = (cu1 - cu0) / max(1, args.steps)
     n_final, _, steps_done = sim.counts()
+    occ = occupied_bins(sim, n_mean)
 
     # the same steps replayed as one captured hipGraph per step (the first
     # step captures and is not timed)
@@ -349,26 +382,17 @@ def main():
     gbatch.close()
     gsim.close()
 
-    # occupied bins of the final state (for the permute/bins byte count)
-    import numpy as np
-    cols = sim.download_cells()
-    dims = np.array(list(desc.dims), dtype=np.int64)
-    vox = np.minimum(np.floor(np.stack(cols[:3], 1) / desc.dx).astype(np.int64), dims - 1)
-    occ_final = len(np.unique(vox[:, 0] + dims[0] * (vox[:, 1] + dims[1] * vox[:, 2])))
-    occ = n_mean * occ_final / max(1, n_final)  # occupied bins per step, scaled to the mean population
-
-    table = {}
-    for name, (us, nl) in phases.items():
-        if nl == 0:
-            continue
+    for name, row in table.items():
         byts = algorithmic_bytes(name, n_mean, w.nvox, len(w.fields), W.CELL_BYTES, occ)
-        avg_us = us / nl
-        gbs = (byts / (avg_us * 1e-6) / 1e9) if byts else None
-        table[name] = {"total_ms": us / 1000.0, "launches": nl, "avg_us": avg_us,
-                       "alg_bytes": byts, "GBps": gbs}
-        _log(f"{name:13s} {us / 1000.0:9.2f} ms {nl:6d} launches {avg_us:9.2f} us/launch"
+        gbs = (byts / (row["avg_us"] * 1e-6) / 1e9) if byts else None
+        row.update({"alg_bytes": byts, "GBps": gbs})
+        _log(f"{name:13s} {row['total_ms']:9.2f} ms {row['launches']:6d} launches {row['avg_us']:9.2f} us/launch"
              + (f" {gbs:8.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:5.1f}% of peak)" if gbs else ""))
-    dom = max((k for k in table if table[k]["alg_bytes"]), key=lambda k: table[k]["total_ms"])
+    dom_avg = dom_us / max(1, dom_n)
+    dom_bytes = algorithmic_bytes(dom, n_mean, w.nvox, len(w.fields), W.CELL_BYTES, occ)
+    dom_gbs = dom_bytes / (dom_avg * 1e-6) / 1e9
+    _log(f"timed region: {value / 1e6:.0f} M cell-updates/s, dominant kernel {dom} {dom_avg:.2f} us/launch "
+         f"{dom_gbs:.0f} GB/s; graph replay {(gcu1 - gcu0) / (g1 - g0) / 1e6:.0f} M cell-updates/s")
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_large_1m_256.json")
     if os.path.exists(tf):
@@ -376,9 +400,9 @@ def main():
             traffic = json.load(open(tf)).get(dom)
         except Exception:
             traffic = None
-    roof = {"bound": "hbm", "kernel": dom, "achieved": table[dom]["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": table[dom]["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
-            "alg_bytes_per_launch": table[dom]["alg_bytes"], "avg_launch_us": table[dom]["avg_us"]}
+    roof = {"bound": "hbm", "kernel": dom, "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic,
+            "alg_bytes_per_launch": dom_bytes, "avg_launch_us": dom_avg}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(w, args.cpu_steps)
     ens = ensemble(ctx, args.ensemble_sims, args.ensemble_steps) if args.ensemble_sims > 0 else None

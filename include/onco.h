@@ -183,11 +183,14 @@ int onc_batch_size(onc_batch* b, int* n);
 /* use_graph: capture one timestep as a hipGraph and replay it nsteps times */
 int onc_batch_step(onc_batch* b, int nsteps, int use_graph);
 int onc_batch_sync(onc_batch* b);
-/* tile shapes: sort/scan items per thread and stencil block (x, y, z-chunk) */
-int onc_batch_set_tuning(onc_batch* b, int sort_items, int stencil_bx, int stencil_by,
-                         int stencil_bz);
-/* per-phase HIP-event timing on the batch stream (profile=1 enables it) */
-int onc_batch_profile(onc_batch* b, int enable);
+/* tile shapes: cells per thread of a radix-sort tile and of a compaction
+   tile (4, 8, 12 or 16), and the stencil block (x, y with x*y == 256, z-chunk) */
+int onc_batch_set_tuning(onc_batch* b, int sort_items, int compact_items, int stencil_bx,
+                         int stencil_by, int stencil_bz);
+/* per-phase HIP-event timing of eager steps on the batch stream: bit k of
+   phase_mask times phase k (0 plan, 1 Morton, 2 sort, 3 permute + bins,
+   4 forces, 5 stencil, 6 rule, 7 compaction, 8 daughters); 0 disables */
+int onc_batch_profile(onc_batch* b, int phase_mask);
 #define ONC_PHASES 10
 int onc_batch_phase_times(onc_batch* b, double* us_total, uint64_t* launches, int n);
 

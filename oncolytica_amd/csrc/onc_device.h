@@ -76,14 +76,17 @@ struct onc_kargs {
     const onc_dsim* sims;
     onc_u32 nsims;
     onc_u32 passes;       // radix passes this step (max over sims)
-    onc_u32 tile;         // cells per sort / compaction tile
-    onc_u32 max_tiles;    // tiles the status arrays hold
-    onc_u32* tile_prefix; // [nsims + 1] tile prefix of live counts
+    onc_u32 tile;         // cells per sort tile
+    onc_u32 max_tiles;    // sort tiles the status array holds
+    onc_u32* tile_prefix; // [nsims + 1] sort tile prefix of live counts
+    onc_u32 ctile;        // cells per compaction tile
+    onc_u32 max_ctiles;   // compaction tiles the status array holds
+    onc_u32* ctile_prefix;// [nsims + 1] compaction tile prefix of live counts
     onc_u32* cell_prefix; // [nsims + 1] prefix of live counts
     onc_u32* hist;        // [nsims][4][256] digit histograms
     onc_u32* counters;    // [16] dynamic tile counters
     onc_u32* sort_status; // [4][max_tiles][256] look-back words
-    onc_u64* comp_status; // [max_tiles] look-back words of the compaction scan
+    onc_u64* comp_status; // [max_ctiles] look-back words of the compaction scan
     onc_u32* jobs_prefix; // [njobs + 1] stencil blocks per (sim, field)
     onc_u32* jobs;        // [njobs] sim << 8 | field
     onc_u32 njobs;

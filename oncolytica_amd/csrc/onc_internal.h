@@ -30,10 +30,11 @@ enum onc_phase {
 #define ONC_PLAN_STEP 0
 #define ONC_PLAN_COMMIT 1
 
-// Launchers (all asynchronous on `s`). `grid_cells` / `grid_tiles` are the
-// persistent grid sizes chosen by the runtime.
+// Launchers (all asynchronous on `s`). Grid sizes are chosen by the runtime;
+// `items` (cells per thread of a tile kernel) is 4, 8, 12 or 16 for the sort
+// passes and 4, 8 or 16 for compaction.
 hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s);
-hipError_t onc_launch_morton(const onc_kargs& a, int items, int grid, hipStream_t s);
+hipError_t onc_launch_morton(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_hist_only(const onc_kargs& a, int items, int grid, hipStream_t s);
 hipError_t onc_launch_sort_pass(const onc_kargs& a, int items, int pass, int grid, hipStream_t s);
 hipError_t onc_launch_permute_bins(const onc_kargs& a, int grid, hipStream_t s);

@@ -149,16 +149,17 @@ __device__ __forceinline__ void lookback_wave64(const onc_u64* status, onc_u32 g
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
     __shared__ onc_u32 s_tmp[8];
-    __shared__ onc_u32 s_carry_t, s_carry_c;
+    __shared__ onc_u32 s_carry_t, s_carry_c, s_carry_ct;
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_carry_t = 0;
         s_carry_c = 0;
+        s_carry_ct = 0;
     }
     __syncthreads();
     for (onc_u32 b0 = 0; b0 < a.nsims; b0 += 256) {
         onc_u32 s = b0 + tid;
-        onc_u32 nt = 0, nc = 0;
+        onc_u32 nt = 0, nc = 0, nct = 0;
         if (s < a.nsims) {
             onc_dstate* st = a.sims[s].state;
             if (st->pending) {
@@ -177,20 +178,24 @@ __global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
                 st->cell_updates += st->n;
                 nc = st->n;
                 nt = (nc + a.tile - 1u) / a.tile;
+                nct = (nc + a.ctile - 1u) / a.ctile;
             }
         }
         if (mode == ONC_PLAN_STEP) {
-            onc_u32 tt, tc;
+            onc_u32 tt, tc, tct;
             onc_u32 et = block_exscan256(nt, s_tmp, &tt);
             onc_u32 ec = block_exscan256(nc, s_tmp, &tc);
+            onc_u32 ect = block_exscan256(nct, s_tmp, &tct);
             if (s < a.nsims) {
                 a.tile_prefix[s] = s_carry_t + et;
                 a.cell_prefix[s] = s_carry_c + ec;
+                a.ctile_prefix[s] = s_carry_ct + ect;
             }
             __syncthreads();
             if (tid == 0) {
                 s_carry_t += tt;
                 s_carry_c += tc;
+                s_carry_ct += tct;
             }
             __syncthreads();
         }
@@ -199,6 +204,7 @@ __global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
         if (tid == 0) {
             a.tile_prefix[a.nsims] = s_carry_t;
             a.cell_prefix[a.nsims] = s_carry_c;
+            a.ctile_prefix[a.nsims] = s_carry_ct;
         }
         for (onc_u32 i = tid; i < a.nsims * 4u * 256u; i += 256) a.hist[i] = 0u;
         for (int i = tid; i < ONC_NCOUNTERS; i += 256) a.counters[i] = 0u;
@@ -206,47 +212,118 @@ __global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
 }
 
 // ---------------------------------------------------------------------------
-// Morton keys + digit histograms of every radix pass (one read of positions);
-// also clears this tile's look-back words for the sort and compaction scans.
+// Morton keys + digit histograms of every radix pass (one read of positions).
+// Each block takes an equal contiguous share of the batch's live cells, in
+// 256-cell chunks, so a small population still spreads over the whole chip;
+// it counts digits in LDS for its current sim and adds the counts to that
+// sim's histogram when its range moves on to the next sim (cells of a chunk
+// that straddles two sims count straight into global memory). The kernel also
+// clears the look-back words of this step's sort and compaction tiles.
 // ---------------------------------------------------------------------------
-template <int ITEMS>
+#define ONC_MORTON_CHUNKS 8
 __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
     __shared__ onc_u32 s_hist[ONC_DEV_MAX_PASSES * 256];
     const int tid = threadIdx.x;
-    const onc_u32 total = a.tile_prefix[a.nsims];
     const onc_u32 P = a.passes;
-    for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
-        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
-        const onc_dsim& S = a.sims[s];
-        const onc_u32 n = S.state->n;
-        const onc_u32 par = S.state->parity;
-        const onc_u32 base = (g - a.tile_prefix[s]) * (onc_u32)(256 * ITEMS);
-        for (onc_u32 i = tid; i < P * 256u; i += 256) s_hist[i] = 0u;
+    const onc_u32 total = a.cell_prefix[a.nsims];
+    const onc_u32 nch = (total + 255u) >> 8;
+    const onc_u32 blk = onc_uniform(onc_xcd_block(blockIdx.x, gridDim.x));
+    const onc_u32 c0 = (onc_u32)(((unsigned long long)nch * blk) / gridDim.x);
+    const onc_u32 c1 = (onc_u32)(((unsigned long long)nch * (blk + 1u)) / gridDim.x);
+    for (onc_u32 i = tid; i < P * 256u; i += 256) s_hist[i] = 0u;
+    __syncthreads();
+    auto flush = [&](int s) {
         __syncthreads();
-        const float* px = (const float*)S.st[par][0];
-        const float* py = (const float*)S.st[par][1];
-        const float* pz = (const float*)S.st[par][2];
-        onc_u32* ko = S.keys[0];
-        onc_u32* vo = S.vals[0];
+        for (onc_u32 i = tid; i < P * 256u; i += 256) {
+            const onc_u32 v = s_hist[i];
+            if (v) atomicAdd(&a.hist[(size_t)s * 4u * 256u + i], v);
+            s_hist[i] = 0u;
+        }
+        __syncthreads();
+    };
+    auto count = [&](int s, int cur, onc_u32 key) {
+        if (s == cur) {
+            for (onc_u32 p = 0; p < P; p++) atomicAdd(&s_hist[p * 256u + ((key >> (8u * p)) & 255u)], 1u);
+        } else {
+            for (onc_u32 p = 0; p < P; p++)
+                atomicAdd(&a.hist[((size_t)s * 4u + p) * 256u + ((key >> (8u * p)) & 255u)], 1u);
+        }
+    };
+    int cur = -1;
+    for (onc_u32 c = c0; c < c1; c += ONC_MORTON_CHUNKS) {
+        const onc_u32 ce = min(c + (onc_u32)ONC_MORTON_CHUNKS, c1);
+        const onc_u32 base = c << 8;
+        const onc_u32 last = min(ce << 8, total) - 1u;
+        const int sa = (int)onc_uniform((onc_u32)onc_find_seg(a.cell_prefix, a.nsims, base));
+        const int sb = (int)onc_uniform((onc_u32)onc_find_seg(a.cell_prefix, a.nsims, last));
+        if (sa == sb) {
+            // up to eight chunks of one sim: all position loads in flight at once
+            if (sa != cur) {
+                if (cur >= 0) flush(cur);
+                cur = sa;
+            }
+            const onc_dsim& S = a.sims[sa];
+            const onc_u32 lo = a.cell_prefix[sa];
+            const onc_u32 par = S.state->parity;
+            const float* px = (const float*)S.st[par][0];
+            const float* py = (const float*)S.st[par][1];
+            const float* pz = (const float*)S.st[par][2];
+            float x[ONC_MORTON_CHUNKS], y[ONC_MORTON_CHUNKS], z[ONC_MORTON_CHUNKS];
 #pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            onc_u32 i = base + (onc_u32)(k * 256 + tid);
-            if (i < n) {
-                onc_u32 key = onc_cell_key(S, px[i], py[i], pz[i]);
-                ko[i] = key;
-                vo[i] = i;
-                for (onc_u32 p = 0; p < P; p++) atomicAdd(&s_hist[p * 256u + ((key >> (8u * p)) & 255u)], 1u);
+            for (int q = 0; q < ONC_MORTON_CHUNKS; q++) {
+                const onc_u32 g = base + (onc_u32)q * 256u + (onc_u32)tid;
+                x[q] = y[q] = z[q] = 0.0f;
+                if (g <= last) {
+                    x[q] = px[g - lo];
+                    y[q] = py[g - lo];
+                    z[q] = pz[g - lo];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < ONC_MORTON_CHUNKS; q++) {
+                const onc_u32 g = base + (onc_u32)q * 256u + (onc_u32)tid;
+                if (g <= last) {
+                    const onc_u32 i = g - lo;
+                    const onc_u32 key = onc_cell_key(S, x[q], y[q], z[q]);
+                    S.keys[0][i] = key;
+                    S.vals[0][i] = i;
+                    count(sa, cur, key);
+                }
+            }
+        } else {
+            // chunks across a sim boundary: one chunk at a time
+            for (onc_u32 cc = c; cc < ce; cc++) {
+                const onc_u32 b0 = cc << 8;
+                const int s0 = (int)onc_uniform((onc_u32)onc_find_seg(a.cell_prefix, a.nsims, b0));
+                if (s0 != cur) {
+                    if (cur >= 0) flush(cur);
+                    cur = s0;
+                }
+                const onc_u32 g = b0 + (onc_u32)tid;
+                if (g < total) {
+                    int s = s0;
+                    if (g >= a.cell_prefix[s0 + 1]) s = onc_find_seg(a.cell_prefix, a.nsims, g);
+                    const onc_dsim& S = a.sims[s];
+                    const onc_u32 i = g - a.cell_prefix[s];
+                    const onc_u32 par = S.state->parity;
+                    const onc_u32 key = onc_cell_key(S, ((const float*)S.st[par][0])[i],
+                                                     ((const float*)S.st[par][1])[i], ((const float*)S.st[par][2])[i]);
+                    S.keys[0][i] = key;
+                    S.vals[0][i] = i;
+                    count(s, cur, key);
+                }
             }
         }
-        __syncthreads();
-        for (onc_u32 p = 0; p < P; p++) {
-            onc_u32 v = s_hist[p * 256u + tid];
-            if (v) atomicAdd(&a.hist[((size_t)s * 4u + p) * 256u + tid], v);
-            a.sort_status[((size_t)p * a.max_tiles + g) * 256u + tid] = 0u;
-        }
-        if (tid == 0) a.comp_status[g] = 0ull;
-        __syncthreads();
     }
+    if (cur >= 0) flush(cur);
+    // look-back words of this step's tiles
+    const onc_u32 nthreads = gridDim.x * 256u;
+    const onc_u32 gid = blockIdx.x * 256u + (onc_u32)tid;
+    const onc_u32 ts = a.tile_prefix[a.nsims] * 256u;
+    for (onc_u32 p = 0; p < P; p++)
+        for (onc_u32 i = gid; i < ts; i += nthreads) a.sort_status[(size_t)p * a.max_tiles * 256u + i] = 0u;
+    const onc_u32 tc = a.ctile_prefix[a.nsims];
+    for (onc_u32 i = gid; i < tc; i += nthreads) a.comp_status[i] = 0ull;
 }
 
 // Histograms only (keys already in keys[0]); used by onc_sort_pairs.
@@ -302,6 +379,10 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
     const onc_u32 shift = 8u * (onc_u32)pass;
     const onc_u64 lt_mask = (1ull << lane) - 1ull;
     onc_u32* status = a.sort_status + (size_t)pass * a.max_tiles * 256u;
+    // Blocks beyond this step's tile count leave without touching the tile
+    // counter: same-address device atomics serialise, and the grid is sized
+    // for the sims' capacity, not their live counts.
+    if (blockIdx.x >= total) return;
     for (;;) {
         if (tid == 0) s_tile = atomicAdd(&a.counters[ONC_CTR_SORT0 + pass], 1u);
         for (int i = tid; i < 4 * 256; i += 256) s_wc[i] = 0u;
@@ -399,9 +480,20 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
     }
 }
 
+// Fixed (Dirichlet) voxel predicate (engine/assets/shaders/tumor_sim.wgsl:420-423).
+__device__ __forceinline__ bool onc_is_fixed(const onc_dfield& f, const int* d, int x, int y, int z) {
+    if (f.bc == 0) return x == 0 || y == 0 || z == 0 || x == d[0] - 1 || y == d[1] - 1 || z == d[2] - 1;
+    if (f.bc != 1 || z != 0) return false;
+    float fx = ((float)x + 0.5f) - (float)d[0] * 0.5f;
+    float fy = ((float)y + 0.5f) - (float)d[1] * 0.5f;
+    float r = f.src_radius;
+    return fx * fx + fy * fy <= r * r;
+}
+
 // ---------------------------------------------------------------------------
-// Gather every state column into sorted order (current -> other buffer) and
-// write bin start/end of every occupied key.
+// Gather every state column into sorted order (current -> other buffer),
+// write bin start/end of every occupied key, and apply the per-voxel uptake
+// once per occupied voxel (tumor_sim.wgsl:387-412 reaction, before the sweep).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
     const onc_u32 P = a.passes;
@@ -415,13 +507,17 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
         const onc_u32 kprev = i > 0u ? keys[i - 1u] : ~k;
         const onc_u32 knext = i + 1u < n ? keys[i + 1u] : ~k;
         // gather in chunks of eight columns: loads first, then stores
+        float px = 0.0f, py = 0.0f, pz = 0.0f;
         for (int c0 = 0; c0 < S.n_cols; c0 += 8) {
             onc_u32 v[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) v[q] = (c0 + q < S.n_cols) ? S.st[par][c0 + q][src] : 0u;
-            if (c0 == 0)
-                ((float4*)S.pos4)[i] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]),
-                                                   __uint_as_float(v[2]), 0.0f);
+            if (c0 == 0) {
+                px = __uint_as_float(v[0]);
+                py = __uint_as_float(v[1]);
+                pz = __uint_as_float(v[2]);
+                ((float4*)S.pos4)[i] = make_float4(px, py, pz, 0.0f);
+            }
 #pragma unroll
             for (int q = 0; q < 8; q++)
                 if (c0 + q < S.n_cols) S.st[par ^ 1u][c0 + q][i] = v[q];
@@ -429,39 +525,42 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
         if (kprev != k) {
             S.bins[(size_t)k * 4u] = i;
             S.bins[(size_t)k * 4u + 2u] = S.state->step + 1u;
+            // uptake, once per occupied voxel, before the sweep: the bin's
+            // first cell counts its run of equal keys (a few cells)
+            onc_u32 cnt = 1u;
+            while (i + cnt < n && keys[i + cnt] == k) cnt++;
+            const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
+                      vz = onc_vox1(pz, S.dx, S.dims[2]);
+            const size_t v = (size_t)vx + (size_t)S.dims[0] * ((size_t)vy + (size_t)S.dims[1] * (size_t)vz);
+            for (int f = 0; f < S.n_fields; f++) {
+                const onc_dfield& fd = S.fields[f];
+                if (fd.uptake == 0.0f || onc_is_fixed(fd, S.dims, vx, vy, vz)) continue;
+                float* u = S.fld[par][f];
+                float w = u[v] - (float)cnt * fd.uptake;
+                u[v] = w < fd.lo ? fd.lo : w;
+            }
         }
         if (knext != k) S.bins[(size_t)k * 4u + 1u] = i + 1u;
     });
 }
 
-// Fixed (Dirichlet) voxel predicate (engine/assets/shaders/tumor_sim.wgsl:420-423).
-__device__ __forceinline__ bool onc_is_fixed(const onc_dfield& f, const int* d, int x, int y, int z) {
-    if (f.bc == 0) return x == 0 || y == 0 || z == 0 || x == d[0] - 1 || y == d[1] - 1 || z == d[2] - 1;
-    if (f.bc != 1 || z != 0) return false;
-    float fx = ((float)x + 0.5f) - (float)d[0] * 0.5f;
-    float fy = ((float)y + 0.5f) - (float)d[1] * 0.5f;
-    float r = f.src_radius;
-    return fx * fx + fy * fy <= r * r;
-}
 
 // ---------------------------------------------------------------------------
 // Neighbour forces over the 27 Moore bins (engine/assets/shaders/
-// tumor_sim.wgsl:505-628 structure; linear spring, linear adhesion taper),
-// explicit overdamped integration, and per-voxel uptake by the first cell of
-// each bin (tumor_sim.wgsl:387-412 reaction, here before the sweep).
+// tumor_sim.wgsl:505-628 structure; linear spring, linear adhesion taper)
+// and explicit overdamped integration. Reads no field, so it runs beside the
+// sweep.
 // ---------------------------------------------------------------------------
 #define ONC_PAIR_LIST 16
 
 __global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
     // per-thread list of interacting neighbours, [slot][thread] (bank-conflict free)
     __shared__ onc_u32 s_pairs[ONC_PAIR_LIST][256];
-    const onc_u32 P = a.passes;
     const int tid = threadIdx.x;
     onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = a.sims[s];
         const onc_u32 par = S.state->parity;
         const onc_u32 epoch = S.state->step + 1u;
-        const onc_u32* __restrict__ keys = S.keys[P & 1u];
         const float4* __restrict__ p4 = (const float4*)S.pos4;
         const float4 me = p4[i];
         const float px = me.x, py = me.y, pz = me.z;
@@ -524,21 +623,6 @@ __global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
         cx[i] = onc_clampf(px + dpx, 0.0f, (float)S.dims[0] * S.dx);
         cy[i] = onc_clampf(py + dpy, 0.0f, (float)S.dims[1] * S.dx);
         cz[i] = onc_clampf(pz + dpz, 0.0f, (float)S.dims[2] * S.dx);
-        // uptake, once per occupied voxel
-        const onc_u32 k = keys[i];
-        if (i == 0u || keys[i - 1u] != k) {
-            const onc_u32 cntb = S.bins[(size_t)k * 4u + 1u] - i;
-            const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
-                      vz = onc_vox1(pz, S.dx, S.dims[2]);
-            const size_t v = (size_t)vx + (size_t)S.dims[0] * ((size_t)vy + (size_t)S.dims[1] * (size_t)vz);
-            for (int f = 0; f < S.n_fields; f++) {
-                const onc_dfield& fd = S.fields[f];
-                if (fd.uptake == 0.0f || onc_is_fixed(fd, S.dims, vx, vy, vz)) continue;
-                float* u = S.fld[par][f];
-                float w = u[v] - (float)cntb * fd.uptake;
-                u[v] = w < fd.lo ? fd.lo : w;
-            }
-        }
     });
 }
 
@@ -785,6 +869,32 @@ __global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
 // dense staging area in parent order. The sim's last tile records totals.
 // ---------------------------------------------------------------------------
 
+// dst[c][slot[k]] = src[c][base + k * 256 + tid] for every column c and
+// every item k whose slot is not NONE.
+template <int ITEMS>
+__device__ __forceinline__ void compact_copy(onc_u32* const* src, onc_u32* const* dst, int n_cols, onc_u32 base,
+                                             const onc_u32* slot) {
+    const int tid = threadIdx.x;
+    for (int c0 = 0; c0 < n_cols; c0 += 4) {
+        onc_u32 v[4][ITEMS];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (c0 + q >= n_cols) break;
+            const onc_u32* __restrict__ sp = src[c0 + q] + base;
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++) v[q][k] = slot[k] != 0xFFFFFFFFu ? sp[k * 256 + tid] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (c0 + q >= n_cols) break;
+            onc_u32* __restrict__ dp = dst[c0 + q];
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++)
+                if (slot[k] != 0xFFFFFFFFu) dp[slot[k]] = v[q][k];
+        }
+    }
+}
+
 template <int ITEMS>
 __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
     constexpr int TILE = 256 * ITEMS;
@@ -794,16 +904,17 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
     __shared__ onc_u32 s_tmp[8];
     __shared__ onc_u32 s_tile, s_pa, s_pd;
     const int tid = threadIdx.x;
-    const onc_u32 total = a.tile_prefix[a.nsims];
+    const onc_u32 total = a.ctile_prefix[a.nsims];
+    if (blockIdx.x >= total) return;  // see k_sort_pass
     for (;;) {
         if (tid == 0) s_tile = atomicAdd(&a.counters[ONC_CTR_COMPACT], 1u);
         __syncthreads();
         const onc_u32 g = onc_uniform(s_tile);
         if (g >= total) break;
-        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
+        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.ctile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
-        const onc_u32 t = g - a.tile_prefix[s];
-        const onc_u32 ntiles_s = a.tile_prefix[s + 1] - a.tile_prefix[s];
+        const onc_u32 t = g - a.ctile_prefix[s];
+        const onc_u32 ntiles_s = a.ctile_prefix[s + 1] - a.ctile_prefix[s];
         const onc_u32 n = S.state->n;
         const onc_u32 base = t * (onc_u32)TILE;
         const onc_u32 cnt = min((onc_u32)TILE, n - base);
@@ -858,9 +969,10 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
             s_dslot[tid * ITEMS + k] = (f & 2u) ? rd++ : NONE;
         }
         __syncthreads();
-        // Column-outer copy: the ITEMS loads of one column are issued together
-        // before any store (stores to one column may alias loads of another,
-        // so an item-outer loop would serialise every load behind a store).
+        // Copies in groups of four columns: all loads of a group are issued
+        // before any of its stores (a store to one column may alias a later
+        // column's load, so the compiler keeps program order and each group
+        // costs one memory round trip instead of one per column).
         onc_u32 sa[ITEMS], sd[ITEMS];
         bool anyd = false;
 #pragma unroll
@@ -870,28 +982,8 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
             sd[k] = j < cnt ? s_dslot[j] : NONE;
             anyd |= sd[k] != NONE;
         }
-        for (int c = 0; c < S.n_cols; c++) {
-            const onc_u32* __restrict__ src = S.st[par][c] + base;
-            onc_u32* __restrict__ dst = S.st[par ^ 1u][c];
-            onc_u32 v[ITEMS];
-#pragma unroll
-            for (int k = 0; k < ITEMS; k++) v[k] = sa[k] != NONE ? src[k * 256 + tid] : 0u;
-#pragma unroll
-            for (int k = 0; k < ITEMS; k++)
-                if (sa[k] != NONE) dst[sa[k]] = v[k];
-        }
-        if (anyd) {
-            for (int c = 0; c < S.n_cols; c++) {
-                const onc_u32* __restrict__ src = S.dau[c] + base;
-                onc_u32* __restrict__ dst = S.dau2[c];
-                onc_u32 v[ITEMS];
-#pragma unroll
-                for (int k = 0; k < ITEMS; k++) v[k] = sd[k] != NONE ? src[k * 256 + tid] : 0u;
-#pragma unroll
-                for (int k = 0; k < ITEMS; k++)
-                    if (sd[k] != NONE) dst[sd[k]] = v[k];
-            }
-        }
+        compact_copy<ITEMS>(S.st[par], S.st[par ^ 1u], S.n_cols, base, sa);
+        if (anyd) compact_copy<ITEMS>(S.dau, S.dau2, S.n_cols, base, sd);
         __syncthreads();
     }
 }
@@ -1054,7 +1146,9 @@ hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s) {
 
 #define ONC_DISPATCH_ITEMS(items, KERNEL, grid, s, ...)                                      \
     do {                                                                                     \
-        if ((items) == 8)                                                                    \
+        if ((items) == 4)                                                                    \
+            hipLaunchKernelGGL(KERNEL<4>, dim3(grid), dim3(256), 0, s, __VA_ARGS__);        \
+        else if ((items) == 8)                                                               \
             hipLaunchKernelGGL(KERNEL<8>, dim3(grid), dim3(256), 0, s, __VA_ARGS__);        \
         else if ((items) == 12)                                                              \
             hipLaunchKernelGGL(KERNEL<12>, dim3(grid), dim3(256), 0, s, __VA_ARGS__);       \
@@ -1062,8 +1156,8 @@ hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s) {
             hipLaunchKernelGGL(KERNEL<16>, dim3(grid), dim3(256), 0, s, __VA_ARGS__);       \
     } while (0)
 
-hipError_t onc_launch_morton(const onc_kargs& a, int items, int grid, hipStream_t s) {
-    ONC_DISPATCH_ITEMS(items, k_morton, grid, s, a);
+hipError_t onc_launch_morton(const onc_kargs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_morton, dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 hipError_t onc_launch_hist_only(const onc_kargs& a, int items, int grid, hipStream_t s) {

@@ -92,20 +92,21 @@ struct onc_batch {
     onc_u32* d_sort_status = nullptr;
     onc_u64* d_comp_status = nullptr;
     onc_u32* d_jobs = nullptr;
-    size_t cap_sims = 0, cap_tiles = 0, cap_jobs = 0;
+    size_t cap_sims = 0, cap_tiles = 0, cap_ctiles = 0, cap_jobs = 0;
     onc_kargs ka{};
     onc_rule_args ra{};
     onc_u32 stencil_blocks = 0;
     uint64_t total_cap = 0;
     int passes = 1;
     // tuning
-    int items = 16;
+    int items = 16;   // cells per thread of a sort tile
+    int citems = 4;   // cells per thread of a compaction tile
     int sbx = 64, sby = 4, sbz = 32;
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // profiling
-    bool profile = false;
+    unsigned profile = 0;  // bit k: time phase k (onc_phase)
     std::vector<hipEvent_t> ev;
     int ev_used = 0;
     double phase_us[ONC_PHASES] = {0};
@@ -590,7 +591,7 @@ static void batch_free_tables(onc_batch* b) {
     b->d_sort_status = nullptr;
     b->d_comp_status = nullptr;
     b->d_jobs = nullptr;
-    b->cap_sims = b->cap_tiles = b->cap_jobs = 0;
+    b->cap_sims = b->cap_tiles = b->cap_ctiles = b->cap_jobs = 0;
 }
 
 static int batch_destroy_internal(onc_batch* b) {
@@ -621,8 +622,8 @@ static int batch_rebuild(onc_batch* b) {
     CK(hipSetDevice(b->ctx->device));
     CK(hipStreamSynchronize(b->stream));
     const size_t S = b->sims.size();
-    const size_t tile = (size_t)256 * b->items;
-    size_t max_tiles = 0;
+    const size_t tile = (size_t)256 * b->items, ctile = (size_t)256 * b->citems;
+    size_t max_tiles = 0, max_ctiles = 0;
     int passes = 1;
     uint64_t total_cap = 0;
     std::vector<onc_dsim> host(S);
@@ -632,6 +633,7 @@ static int batch_rebuild(onc_batch* b) {
         onc_sim* s = b->sims[i];
         host[i] = s->dsim;
         max_tiles += (s->desc.capacity + tile - 1) / tile;
+        max_ctiles += (s->desc.capacity + ctile - 1) / ctile;
         passes = std::max(passes, (s->key_bits + 7) / 8);
         total_cap += s->desc.capacity;
         const int X = s->desc.dims[0], Y = s->desc.dims[1], Z = s->desc.dims[2];
@@ -643,32 +645,34 @@ static int batch_rebuild(onc_batch* b) {
         }
     }
     if (max_tiles == 0) max_tiles = 1;
+    if (max_ctiles == 0) max_ctiles = 1;
     jprefix.push_back(blocks);
     // (re)allocate
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
-    if (S > b->cap_sims || max_tiles > b->cap_tiles || jobs.size() + 1 > b->cap_jobs) {
+    if (S > b->cap_sims || max_tiles > b->cap_tiles || max_ctiles > b->cap_ctiles || jobs.size() + 1 > b->cap_jobs) {
         batch_free_tables(b);
-        size_t cs = std::max<size_t>(S, 1), ct = max_tiles, cj = jobs.size() + 1;
+        size_t cs = std::max<size_t>(S, 1), ct = max_tiles, cct = max_ctiles, cj = jobs.size() + 1;
         void* p;
         if (hipMalloc(&p, cs * sizeof(onc_dsim)) != hipSuccess) return fail(ONC_ERR_NOMEM, "batch table");
         b->d_sims = (onc_dsim*)p;
-        if (hipMalloc(&p, (2 * (cs + 1) + ONC_NCOUNTERS + 16) * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "batch small");
+        if (hipMalloc(&p, (3 * (cs + 1) + ONC_NCOUNTERS + 16) * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "batch small");
         b->d_small = (onc_u32*)p;
         if (hipMalloc(&p, cs * 4 * 256 * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "batch hist");
         b->d_hist = (onc_u32*)p;
         if (hipMalloc(&p, (size_t)ONC_DEV_MAX_PASSES * ct * 256 * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "sort status");
         b->d_sort_status = (onc_u32*)p;
-        if (hipMalloc(&p, ct * 8) != hipSuccess) return fail(ONC_ERR_NOMEM, "compaction status");
+        if (hipMalloc(&p, cct * 8) != hipSuccess) return fail(ONC_ERR_NOMEM, "compaction status");
         b->d_comp_status = (onc_u64*)p;
         if (hipMalloc(&p, 2 * cj * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "stencil jobs");
         b->d_jobs = (onc_u32*)p;
         b->cap_sims = cs;
         b->cap_tiles = ct;
+        b->cap_ctiles = cct;
         b->cap_jobs = cj;
-        CK(hipMemsetAsync(b->d_small, 0, (2 * (cs + 1) + ONC_NCOUNTERS + 16) * 4, b->stream));
+        CK(hipMemsetAsync(b->d_small, 0, (3 * (cs + 1) + ONC_NCOUNTERS + 16) * 4, b->stream));
     }
     if (S) CK(hipMemcpyAsync(b->d_sims, host.data(), S * sizeof(onc_dsim), hipMemcpyHostToDevice, b->stream));
     std::vector<onc_u32> jj(jobs);
@@ -684,9 +688,12 @@ static int batch_rebuild(onc_batch* b) {
     a.passes = (onc_u32)passes;
     a.tile = (onc_u32)tile;
     a.max_tiles = (onc_u32)max_tiles;
+    a.ctile = (onc_u32)ctile;
+    a.max_ctiles = (onc_u32)max_ctiles;
     a.tile_prefix = b->d_small;
     a.cell_prefix = b->d_small + (b->cap_sims + 1);
-    a.counters = b->d_small + 2 * (b->cap_sims + 1);
+    a.ctile_prefix = b->d_small + 2 * (b->cap_sims + 1);
+    a.counters = b->d_small + 3 * (b->cap_sims + 1);
     a.err = a.counters + ONC_NCOUNTERS;
     a.hist = b->d_hist;
     a.sort_status = b->d_sort_status;
@@ -723,9 +730,23 @@ static int grid_cells(onc_batch* b) {
     uint64_t lim = (uint64_t)b->ctx->cus * 8;
     return (int)std::max<uint64_t>(1, std::min(need, lim));
 }
+// Tile kernels: as many resident blocks per CU as their LDS allows (smaller
+// tiles, more blocks).
 static int grid_tiles(onc_batch* b) {
     uint64_t need = b->ka.max_tiles;
-    uint64_t lim = (uint64_t)b->ctx->cus * 4;
+    uint64_t lim = (uint64_t)b->ctx->cus * (64 / b->items);
+    return (int)std::max<uint64_t>(1, std::min(need, lim));
+}
+static int grid_ctiles(onc_batch* b) {
+    uint64_t need = b->ka.max_ctiles;
+    uint64_t lim = (uint64_t)b->ctx->cus * (64 / b->citems);
+    return (int)std::max<uint64_t>(1, std::min(need, lim));
+}
+// Morton: one block per CU at most (each holds an LDS histogram that it adds
+// to global memory once per sim it touches).
+static int grid_morton(onc_batch* b) {
+    uint64_t need = (b->total_cap + 255) / 256;
+    uint64_t lim = (uint64_t)b->ctx->cus;
     return (int)std::max<uint64_t>(1, std::min(need, lim));
 }
 
@@ -740,15 +761,21 @@ static int launch_rule(onc_batch* b) {
     return ONC_OK;
 }
 
-// Profiling: 10 events per step from a pool, recorded between the phases of
-// eager (non-graph) steps; durations are accumulated at the next sync, so the
-// host never waits inside a run of steps.
+// Profiling: ONC_EV_STEP events per step from a pool, recorded between the
+// phases of eager (non-graph) steps; durations are accumulated at the next
+// sync, so the host never waits inside a run of steps. Slot k is recorded
+// before phase k and slot k + 1 after it, only for the phases being timed (an
+// event costs a few microseconds of GPU time, so timing one phase leaves the
+// others nearly undisturbed).
+#define ONC_EV_STEP 10
 static void prof_mark(onc_batch* b, int slot) {
-    hipEventRecord(b->ev[b->ev_used * 10 + slot], b->stream);
+    const unsigned m = b->profile;
+    const bool want = (slot < ONC_EV_STEP - 1 && ((m >> slot) & 1u)) || (slot > 0 && ((m >> (slot - 1)) & 1u));
+    if (want) hipEventRecord(b->ev[b->ev_used * ONC_EV_STEP + slot], b->stream);
 }
 
 static int prof_reserve(onc_batch* b, int nsteps) {
-    size_t need = (size_t)(b->ev_used + nsteps) * 10;
+    size_t need = (size_t)(b->ev_used + nsteps) * ONC_EV_STEP;
     while (b->ev.size() < need) {
         hipEvent_t e;
         CK(hipEventCreate(&e));
@@ -759,9 +786,11 @@ static int prof_reserve(onc_batch* b, int nsteps) {
 
 static int prof_collect(onc_batch* b) {
     for (int k = 0; k < b->ev_used; k++) {
-        for (int ph = 0; ph < 9; ph++) {
+        hipEvent_t* e = &b->ev[(size_t)k * ONC_EV_STEP];
+        for (int ph = 0; ph < ONC_EV_STEP - 1; ph++) {
+            if (!((b->profile >> ph) & 1u)) continue;
             float ms = 0.0f;
-            CK(hipEventElapsedTime(&ms, b->ev[k * 10 + ph], b->ev[k * 10 + ph + 1]));
+            CK(hipEventElapsedTime(&ms, e[ph], e[ph + 1]));
             b->phase_us[ph] += 1000.0 * ms;
             b->phase_n[ph] += (ph == ONC_PH_SORT) ? (uint64_t)b->passes : 1u;
         }
@@ -770,7 +799,9 @@ static int prof_collect(onc_batch* b) {
     return ONC_OK;
 }
 
-// Enqueue one timestep for every sim of the batch.
+
+// Enqueue one timestep for every sim of the batch. Uptake is applied by the
+// permute kernel, so neither the forces nor the sweep depends on the other.
 static int enqueue_step(onc_batch* b, bool profiling) {
     const onc_kargs& a = b->ka;
     hipStream_t st = b->stream;
@@ -778,7 +809,7 @@ static int enqueue_step(onc_batch* b, bool profiling) {
     if (profiling) prof_mark(b, 0);
     CK(onc_launch_plan(a, ONC_PLAN_STEP, st));
     if (profiling) prof_mark(b, 1);
-    CK(onc_launch_morton(a, b->items, gt, st));
+    CK(onc_launch_morton(a, grid_morton(b), st));
     if (profiling) prof_mark(b, 2);
     for (int p = 0; p < b->passes; p++) CK(onc_launch_sort_pass(a, b->items, p, gt, st));
     if (profiling) prof_mark(b, 3);
@@ -790,7 +821,7 @@ static int enqueue_step(onc_batch* b, bool profiling) {
     if (profiling) prof_mark(b, 6);
     CKR(launch_rule(b));
     if (profiling) prof_mark(b, 7);
-    CK(onc_launch_compact(a, b->items, gt, st));
+    CK(onc_launch_compact(a, b->citems, grid_ctiles(b), st));
     if (profiling) prof_mark(b, 8);
     CK(onc_launch_daughters(a, gc, st));
     if (profiling) prof_mark(b, 9);
@@ -939,14 +970,18 @@ extern "C" int onc_batch_sync(onc_batch* b) {
     return batch_sync_impl(b);
 }
 
-extern "C" int onc_batch_set_tuning(onc_batch* b, int items, int bx, int by, int bz) {
+extern "C" int onc_batch_set_tuning(onc_batch* b, int items, int citems, int bx, int by, int bz) {
     if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
-    if (items != 8 && items != 12 && items != 16) return fail(ONC_ERR_INVALID, "items must be 8, 12 or 16");
+    if (items != 4 && items != 8 && items != 12 && items != 16)
+        return fail(ONC_ERR_INVALID, "sort items must be 4, 8, 12 or 16");
+    if (citems != 4 && citems != 8 && citems != 12 && citems != 16)
+        return fail(ONC_ERR_INVALID, "compaction items must be 4, 8, 12 or 16");
     if (bx < 1 || by < 1 || bx * by != 256 || bz < 1 || bz > 1024)
         return fail(ONC_ERR_INVALID, "stencil block must satisfy bx*by == 256, 1 <= bz <= 1024");
     CK(hipSetDevice(b->ctx->device));
     CK(hipStreamSynchronize(b->stream));
     b->items = items;
+    b->citems = citems;
     b->sbx = bx;
     b->sby = by;
     b->sbz = bz;
@@ -954,12 +989,13 @@ extern "C" int onc_batch_set_tuning(onc_batch* b, int items, int bx, int by, int
     return ONC_OK;
 }
 
-extern "C" int onc_batch_profile(onc_batch* b, int enable) {
+extern "C" int onc_batch_profile(onc_batch* b, int phase_mask) {
+    const int enable = phase_mask;
     if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
     CK(hipSetDevice(b->ctx->device));
     CK(hipStreamSynchronize(b->stream));
     b->ev_used = 0;
-    b->profile = enable != 0;
+    b->profile = (unsigned)enable & ((1u << (ONC_EV_STEP - 1)) - 1u);
     for (int i = 0; i < ONC_PHASES; i++) {
         b->phase_us[i] = 0.0;
         b->phase_n[i] = 0;
@@ -1023,7 +1059,7 @@ extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* v
     };
     size_t o_k0 = take(n * 4ull), o_k1 = take(n * 4ull), o_v0 = take(n * 4ull), o_v1 = take(n * 4ull);
     size_t o_sim = take(sizeof(onc_dsim)), o_st = take(sizeof(onc_dstate));
-    size_t o_small = take((2 * 2 + ONC_NCOUNTERS + 16) * 4);
+    size_t o_small = take((3 * 2 + ONC_NCOUNTERS + 16) * 4);
     size_t o_hist = take(4 * 256 * 4);
     size_t o_ss = take((size_t)ONC_DEV_MAX_PASSES * tiles * 256 * 4);
     size_t o_cs = take(tiles * 8);
@@ -1042,7 +1078,7 @@ extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* v
     hipStream_t st = c->stream;
     CK(hipMemcpyAsync(B + o_sim, &ds, sizeof(ds), hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(B + o_st, &dst, sizeof(dst), hipMemcpyHostToDevice, st));
-    CK(hipMemsetAsync(B + o_small, 0, (2 * 2 + ONC_NCOUNTERS + 16) * 4, st));
+    CK(hipMemsetAsync(B + o_small, 0, (3 * 2 + ONC_NCOUNTERS + 16) * 4, st));
     CK(hipMemcpyAsync(ds.keys[0], kin, n * 4ull, hipMemcpyDeviceToDevice, st));
     CK(hipMemcpyAsync(ds.vals[0], vin, n * 4ull, hipMemcpyDeviceToDevice, st));
     onc_kargs a{};
@@ -1052,9 +1088,12 @@ extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* v
     a.tile = (onc_u32)tile;
     a.max_tiles = (onc_u32)tiles;
     onc_u32* small = (onc_u32*)(B + o_small);
+    a.ctile = (onc_u32)tile;
+    a.max_ctiles = (onc_u32)tiles;
     a.tile_prefix = small;
     a.cell_prefix = small + 2;
-    a.counters = small + 4;
+    a.ctile_prefix = small + 4;
+    a.counters = small + 6;
     a.err = a.counters + ONC_NCOUNTERS;
     a.hist = (onc_u32*)(B + o_hist);
     a.sort_status = (onc_u32*)(B + o_ss);

@@ -1,19 +1,21 @@
 """Autotuner for a batch's tile shapes.
 
-Two independent knobs of `onc_batch_set_tuning`:
+Three independent knobs of `onc_batch_set_tuning`:
 
-* items per thread of the tile kernels (Morton keys + digit histograms, the
-  onesweep radix passes and the compaction scan): 8, 12 or 16, i.e. tiles of
-  2048, 3072 or 4096 cells. Larger tiles mean fewer look-back predecessors,
-  smaller ones more tiles in flight for small sims.
+* cells per thread of a radix-sort tile (the onesweep passes): 4, 8, 12 or
+  16, i.e. tiles of 1024 to 4096 cells. Larger tiles mean fewer look-back
+  predecessors and fewer status words, smaller ones more tiles in flight
+  (each tile's block works on one CU, so a small population needs small
+  tiles to use the chip).
+* cells per thread of a compaction tile, the same choice for the compaction
+  scan and its column copies.
 * the stencil block: `sbx` float4 columns (a power of two up to 256) by
   256 / sbx rows, marching `sbz` planes along z.
 
 Each candidate runs a few eager steps of throwaway copies of the batch's
 sims, re-uploaded to the same initial state before every candidate, with HIP
-events around every phase; items is chosen by the summed time of the Morton,
-sort and compaction phases, the stencil block by the stencil phase alone
-(the two knobs touch disjoint kernels). Results are cached per device
+events around every phase; each knob is chosen by the time of the phase it
+shapes (the knobs touch disjoint kernels). Results are cached per device
 architecture and batch signature, in memory and optionally in a JSON file.
 Every shape computes the same result bit for bit (tested); only time moves.
 """
@@ -30,10 +32,9 @@ import numpy as np
 from . import _abi
 from ._sim import Batch, Sim
 
-ITEMS = (8, 12, 16)
+ITEMS = (4, 8, 16)
 STENCIL_SBX = (16, 32, 64, 128)
 STENCIL_SBZ = (8, 16, 32, 64)
-TILE_PHASES = ("morton", "sort", "compact")
 
 _memo: dict[str, "Tuning"] = {}
 
@@ -41,12 +42,13 @@ _memo: dict[str, "Tuning"] = {}
 @dataclass(frozen=True)
 class Tuning:
     items: int = 16
+    citems: int = 4
     sbx: int = 64
     sby: int = 4
     sbz: int = 32
 
     def apply(self, batch: Batch) -> None:
-        batch.set_tuning(self.items, self.sbx, self.sby, self.sbz)
+        batch.set_tuning(self.items, self.citems, self.sbx, self.sby, self.sbz)
 
 
 def signature(arch: str, descs: Sequence[_abi.SimDesc]) -> str:
@@ -65,7 +67,7 @@ def _phase_us(batch: Batch, names: Sequence[str]) -> float:
 def tune(ctx: _abi.Context, members: Sequence[tuple[_abi.SimDesc, Sequence[np.ndarray]]], steps: int = 3,
          cache_file: Optional[str] = None, log=None) -> Tuning:
     """Tile shapes for a batch of `members` = [(descriptor, initial cell
-    columns)]. Runs len(ITEMS) + len(STENCIL_SBX) * len(STENCIL_SBZ)
+    columns)]. Runs 2 len(ITEMS) + len(STENCIL_SBX) len(STENCIL_SBZ)
     candidates of `steps` eager steps each on throwaway sims."""
     arch = ctx.info().arch.decode()
     key = signature(arch, [d for d, _ in members])
@@ -92,7 +94,7 @@ def tune(ctx: _abi.Context, members: Sequence[tuple[_abi.SimDesc, Sequence[np.nd
                 s.upload_cells(cols)
             batch.step(1, graph=False)  # first launches after a rebuild
             batch.sync()
-            batch.profile(True)
+            batch.profile(True, phases=names)
             batch.step(steps, graph=False)
             batch.sync()
             us = _phase_us(batch, names)
@@ -102,20 +104,27 @@ def tune(ctx: _abi.Context, members: Sequence[tuple[_abi.SimDesc, Sequence[np.nd
         base = Tuning()
         best_items, best_t = base.items, math.inf
         for it in ITEMS:
-            us = run(Tuning(it, base.sbx, base.sby, base.sbz), TILE_PHASES)
+            us = run(Tuning(it, base.citems, base.sbx, base.sby, base.sbz), ("sort",))
             if log:
-                log(f"autotune items={it}: {us / steps:.1f} us/step (morton+sort+compact)")
+                log(f"autotune sort items={it}: {us / steps:.1f} us/step")
             if us < best_t:
                 best_items, best_t = it, us
+        best_citems, best_t = base.citems, math.inf
+        for it in ITEMS:
+            us = run(Tuning(best_items, it, base.sbx, base.sby, base.sbz), ("compact",))
+            if log:
+                log(f"autotune compaction items={it}: {us / steps:.1f} us/step")
+            if us < best_t:
+                best_citems, best_t = it, us
         best_s, best_t = (base.sbx, base.sbz), math.inf
         for sbx in STENCIL_SBX:
             for sbz in STENCIL_SBZ:
-                us = run(Tuning(best_items, sbx, 256 // sbx, sbz), ("stencil",))
+                us = run(Tuning(best_items, best_citems, sbx, 256 // sbx, sbz), ("stencil",))
                 if log:
                     log(f"autotune stencil sbx={sbx} sby={256 // sbx} sbz={sbz}: {us / steps:.1f} us/step")
                 if us < best_t:
                     best_s, best_t = (sbx, sbz), us
-        t = Tuning(best_items, best_s[0], 256 // best_s[0], best_s[1])
+        t = Tuning(best_items, best_citems, best_s[0], 256 // best_s[0], best_s[1])
     finally:
         batch.close()
         for s in sims:

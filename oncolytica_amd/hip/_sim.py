@@ -137,8 +137,8 @@ class Batch:
         call("onc_batch_remove", self.h, sim.h)
         self.sims.remove(sim)
 
-    def set_tuning(self, items: int, bx: int, by: int, bz: int) -> None:
-        call("onc_batch_set_tuning", self.h, items, bx, by, bz)
+    def set_tuning(self, items: int, citems: int, bx: int, by: int, bz: int) -> None:
+        call("onc_batch_set_tuning", self.h, items, citems, bx, by, bz)
 
     def step(self, n: int = 1, graph: bool = True) -> None:
         call("onc_batch_step", self.h, n, 1 if graph else 0)
@@ -146,8 +146,18 @@ class Batch:
     def sync(self) -> None:
         call("onc_batch_sync", self.h)
 
-    def profile(self, enable: bool = True) -> None:
-        call("onc_batch_profile", self.h, 1 if enable else 0)
+    def profile(self, enable: bool = True, phases=None) -> None:
+        """Time phases of eager steps with HIP events: all of them, or only
+        the named ones (each event costs GPU time of its own)."""
+        if not enable:
+            mask = 0
+        elif phases is None:
+            mask = (1 << 9) - 1
+        else:
+            mask = 0
+            for p in phases:
+                mask |= 1 << _abi.PHASE_NAMES.index(p)
+        call("onc_batch_profile", self.h, mask)
 
     def phase_times(self) -> dict[str, tuple[float, int]]:
         us = (C.c_double * _abi.PHASES)()

@@ -48,7 +48,8 @@ def _run_batch(ctx, ws, tuning, steps):
     return out
 
 
-@pytest.mark.parametrize("tuning", [Tuning(8, 16, 16, 8), Tuning(12, 128, 2, 64), Tuning(16, 256, 1, 1)])
+@pytest.mark.parametrize("tuning", [Tuning(4, 16, 16, 16, 8), Tuning(12, 8, 128, 2, 64), Tuning(16, 4, 256, 1, 1),
+                                    Tuning(8, 12, 64, 4, 32)])
 def test_tile_shapes_do_not_change_results(ctx, tuning):
     ws = _small_ensemble()
     ref = _run_batch(ctx, ws, None, 10)
@@ -62,14 +63,14 @@ def test_autotuner_returns_a_valid_shape_and_caches(ctx, tmp_path):
     members = [(w.desc(), W.initial_cells(w)) for w in ws]
     cache = str(tmp_path / "tune.json")
     t = tune(ctx, members, steps=1, cache_file=cache)
-    assert t.items in (8, 12, 16) and t.sbx * t.sby == 256 and t.sbz >= 1
+    assert t.items in (4, 8, 12, 16) and t.citems in (4, 8, 12, 16) and t.sbx * t.sby == 256 and t.sbz >= 1
     assert tune(ctx, members, steps=1, cache_file=cache) == t
 
 
 def test_set_tuning_rejects_bad_shapes(ctx):
     b = Batch(ctx)
     try:
-        for args in ((10, 64, 4, 32), (16, 64, 3, 32), (16, 64, 4, 0)):
+        for args in ((10, 4, 64, 4, 32), (16, 5, 64, 4, 32), (16, 4, 64, 3, 32), (16, 4, 64, 4, 0)):
             with pytest.raises(_abi.OncError):
                 b.set_tuning(*args)
     finally:
