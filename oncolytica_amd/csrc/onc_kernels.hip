@@ -373,22 +373,20 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
     __shared__ onc_u32 s_texcl[256];
     __shared__ onc_u32 s_gbase[256];
     __shared__ onc_u32 s_tmp[8];
-    __shared__ onc_u32 s_tile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const onc_u32 total = a.tile_prefix[a.nsims];
     const onc_u32 shift = 8u * (onc_u32)pass;
     const onc_u64 lt_mask = (1ull << lane) - 1ull;
     onc_u32* status = a.sort_status + (size_t)pass * a.max_tiles * 256u;
-    // Blocks beyond this step's tile count leave without touching the tile
-    // counter: same-address device atomics serialise, and the grid is sized
-    // for the sims' capacity, not their live counts.
-    if (blockIdx.x >= total) return;
-    for (;;) {
-        if (tid == 0) s_tile = atomicAdd(&a.counters[ONC_CTR_SORT0 + pass], 1u);
+    // Static tile order: block b takes tiles b, b + grid, ... A tile only
+    // waits (look-back) on lower tiles, which belong to blocks dispatched
+    // before it, and every block walks its tiles in increasing order, so the
+    // lowest unfinished tile can always run. (A shared tile counter would
+    // also give that order, but same-address device atomics serialise at one
+    // memory channel and stall every other access queued behind them.)
+    for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
         for (int i = tid; i < 4 * 256; i += 256) s_wc[i] = 0u;
         __syncthreads();
-        const onc_u32 g = onc_uniform(s_tile);
-        if (g >= total) break;
         const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.tile_prefix[s];
@@ -902,15 +900,10 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
     __shared__ onc_u32 s_aslot[TILE];
     __shared__ onc_u32 s_dslot[TILE];
     __shared__ onc_u32 s_tmp[8];
-    __shared__ onc_u32 s_tile, s_pa, s_pd;
+    __shared__ onc_u32 s_pa, s_pd;
     const int tid = threadIdx.x;
     const onc_u32 total = a.ctile_prefix[a.nsims];
-    if (blockIdx.x >= total) return;  // see k_sort_pass
-    for (;;) {
-        if (tid == 0) s_tile = atomicAdd(&a.counters[ONC_CTR_COMPACT], 1u);
-        __syncthreads();
-        const onc_u32 g = onc_uniform(s_tile);
-        if (g >= total) break;
+    for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {  // static tile order, as in k_sort_pass
         const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.ctile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.ctile_prefix[s];

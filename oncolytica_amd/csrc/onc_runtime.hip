@@ -730,17 +730,13 @@ static int grid_cells(onc_batch* b) {
     uint64_t lim = (uint64_t)b->ctx->cus * 8;
     return (int)std::max<uint64_t>(1, std::min(need, lim));
 }
-// Tile kernels: as many resident blocks per CU as their LDS allows (smaller
-// tiles, more blocks).
+// Tile kernels: one block per tile the sims' capacity can need (blocks past
+// this step's live tiles leave at once), up to a cap beyond which blocks loop.
 static int grid_tiles(onc_batch* b) {
-    uint64_t need = b->ka.max_tiles;
-    uint64_t lim = (uint64_t)b->ctx->cus * (64 / b->items);
-    return (int)std::max<uint64_t>(1, std::min(need, lim));
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(b->ka.max_tiles, 65535));
 }
 static int grid_ctiles(onc_batch* b) {
-    uint64_t need = b->ka.max_ctiles;
-    uint64_t lim = (uint64_t)b->ctx->cus * (64 / b->citems);
-    return (int)std::max<uint64_t>(1, std::min(need, lim));
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(b->ka.max_ctiles, 65535));
 }
 // Morton: one block per CU at most (each holds an LDS histogram that it adds
 // to global memory once per sim it touches).
