@@ -55,23 +55,45 @@ def algorithmic_bytes(phase: str, n_mean: float, nvox: int, n_fields: int, cell_
     }.get(phase)
 
 
-def cpu_baseline(w, steps: int):
-    """The oracle (binary32 C, one thread) on `steps` steps of the workload
-    from its seeded initial state."""
+def cpu_baseline(w, steps: int, steps_1t: int):
+    """The oracle (binary32 C restatement; its per-cell and per-voxel loops
+    run on all host cores with OpenMP) on `steps` steps of the workload from
+    its seeded initial state, and the same on one thread for `steps_1t`
+    steps."""
     from oracle import oracle as O
     from oncolytica_amd import workloads as W
     cols = W.initial_cells(w)
-    st = O.TumorState(w.desc(), cols)
-    cu = 0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        cu += st.n.value
-        st.run(1)
-    dt = time.perf_counter() - t0
-    return {"value": cu / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+
+    def rate(nsteps):
+        st = O.TumorState(w.desc(), cols)
+        cu = 0
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            cu += st.n.value
+            st.run(1)
+        return cu / (time.perf_counter() - t0), time.perf_counter() - t0
+
+    threads = O.threads()
+    v, dt = rate(steps)
+    O.set_threads(1)
+    v1, dt1 = rate(steps_1t)
+    O.set_threads(threads)
+    return {"value": v, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of {w.name} from its seeded initial state "
                       f"({w.n_cells:,} cells, {w.dims[0]}^3, {len(w.fields)} fields), "
-                      f"oracle/onco_oracle_real.c binary32, one thread, {dt:.1f} s"}
+                      f"oracle/onco_oracle_real.c binary32 on {threads} OpenMP threads, {dt:.1f} s",
+            "value_1thread": v1, "sample_1thread": f"{steps_1t} steps on 1 thread, {dt1:.1f} s",
+            "host_cpu": _cpu_model()}
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _sim_cu(sim) -> int:
@@ -160,6 +182,124 @@ def serve(ctx, rate: float, seconds: float, budget_gb: float):
     out.update({"workload": "served_poisson", "rate_jobs_per_s": rate, "arrival_window_s": seconds})
     _log(f"serving: p50 {out['p50_latency_s']:.2f} s p99 {out['p99_latency_s']:.2f} s, "
          f"{out['cell_updates_per_s'] / 1e6:.0f} M cell-updates/s, peak {out['peak_resident_jobs']} resident")
+    return out
+
+
+def _drift(a64, b32):
+    """Difference of binary32 results from binary64 ones after one step from
+    the same state: max absolute, and relative to max(|x64|, 1e-3): max, RMS,
+    99.9th percentile and the fraction of elements above 1e-5."""
+    import numpy as np
+    a = np.asarray(a64, dtype=np.float64)
+    b = np.asarray(b32, dtype=np.float64)
+    if a.size == 0:
+        return {"max_abs": 0.0, "max_rel": 0.0, "rms_rel": 0.0, "p999_rel": 0.0, "frac_rel_gt_1e-5": 0.0}
+    d = np.abs(a - b)
+    r = d / np.maximum(np.abs(a), 1e-3)
+    return {"max_abs": float(d.max()), "max_rel": float(r.max()), "rms_rel": float(np.sqrt(np.mean(r * r))),
+            "p999_rel": float(np.percentile(r, 99.9)), "frac_rel_gt_1e-5": float(np.mean(r > 1e-5))}
+
+
+def stress(ctx, steps: int = 300, burst=(100, 110), checkpoints=(0, 105)):
+    """Stress workload: 16M cells in 32 Gaussian clusters, 512^3, three
+    fields, 300 steps with a mitotic burst (division age 1 for steps
+    [100, 110)). Timed as one captured graph per step (re-captured when the
+    parameters change). At each checkpoint k a fresh run is stepped to k,
+    downloaded, stepped once more and downloaded again; the oracle takes one
+    step from the same step-k state in binary32 (bit-exact parity at full
+    size) and in binary64 (per-step drift of positions, matched by cell id,
+    and of fields)."""
+    import numpy as np
+    from oncolytica_amd import workloads as W
+    from oncolytica_amd.hip._sim import Batch, Sim
+    from oracle import oracle as O
+    w = W.stress_c5()
+    cols0 = W.initial_cells(w)
+    base = list(w.params)
+    hot = [1.0] + base[1:]
+    _log(f"stress: {w.n_cells:,} cells, {w.dims[0]}^3, {len(w.fields)} fields, {steps} steps, burst {burst}")
+
+    def params_at(k):
+        return hot if burst[0] <= k < burst[1] else base
+
+    def run_to(k, timed=False):
+        sim = Sim(ctx, w.desc())
+        sim.upload_cells(cols0)
+        b = Batch(ctx)
+        b.add(sim)
+        b.sync()
+        done, ns = 0, []
+        t0 = time.perf_counter()
+        for edge in (burst[0], burst[1], k):
+            if edge <= done or done >= k:
+                continue
+            e = min(edge, k)
+            sim.set_params(params_at(done))
+            b.step(e - done, graph=True)
+            done = e
+            if timed:
+                b.sync()
+                ns.append((done, sim.counts()[0]))
+        b.sync()
+        return sim, b, time.perf_counter() - t0, ns
+
+    sim, b, wall, ns = run_to(steps, timed=True)
+    cu = _sim_cu(sim)
+    n_final = sim.counts()[0]
+    b.close()
+    sim.close()
+    out = {"workload": w.name, "cells_initial": w.n_cells, "grid": list(w.dims), "fields": len(w.fields),
+           "steps": steps, "burst_steps": list(burst), "wall_s": wall, "cell_updates": cu,
+           "cell_updates_per_s": cu / wall, "live_cells_at": ns, "final_live_cells": n_final,
+           "checkpoints": []}
+    _log(f"stress: {cu / wall / 1e6:.0f} M cell-updates/s over {steps} steps ({wall:.2f} s), n at {ns}")
+    for k in checkpoints:
+        sim, b, _, _ = run_to(k)
+        ck = sim.download_cells()
+        fk = [sim.download_field(f) for f in range(len(w.fields))]
+        nid = sim.counts()[1]
+        sim.set_params(params_at(k))
+        b.step(1, graph=False)
+        b.sync()
+        g1 = sim.download_cells()
+        gf1 = [sim.download_field(f) for f in range(len(w.fields))]
+        b.close()
+        sim.close()
+        d = w.desc()
+        for i, p in enumerate(params_at(k)):
+            d.params[i] = p
+        t0 = time.perf_counter()
+        o32 = O.TumorState(d, ck, fields=fk, real="f32", next_id=nid)
+        o32.step.value = k
+        o32.run(1)
+        c32 = o32.columns()
+        same = len(c32[0]) == len(g1[0]) and all(
+            np.array_equal(a.view(np.uint32), g.view(np.uint32)) for a, g in zip(c32, g1)) and all(
+            np.array_equal(a.view(np.uint32), g.view(np.uint32)) for a, g in zip(o32.fields, gf1))
+        del o32
+        o64 = O.TumorState(d, ck, fields=fk, real="f64", next_id=nid)
+        o64.step.value = k
+        o64.run(1)
+        c64 = o64.columns()
+        # match the cells that existed at step k by id (a threshold decision
+        # can differ between precisions, and then the ids of this step's
+        # daughters, assigned by rank, no longer name the same cells)
+        ids32, ids64 = g1[3], c64[3]
+        common, i32, i64 = np.intersect1d(ids32, ids64, assume_unique=True, return_indices=True)
+        old_cells = common < nid
+        i32, i64 = i32[old_cells], i64[old_cells]
+        pos = _drift(np.concatenate([c64[c][i64] for c in range(3)]), np.concatenate([g1[c][i32] for c in range(3)]))
+        fld = [_drift(o64.fields[f], gf1[f]) for f in range(len(w.fields))]
+        del o64
+        ck_out = {"step": k, "cells": int(len(ck[0])), "bit_identical_to_oracle_f32": bool(same),
+                  "live_after_f32": int(len(ids32)), "live_after_f64": int(len(ids64)),
+                  "daughters_f32": int(np.sum(ids32 >= nid)), "daughters_f64": int(np.sum(ids64 >= nid)),
+                  "survivor_mismatches_vs_f64": int(np.sum(ids32 < nid) + np.sum(ids64 < nid) - 2 * len(i32)),
+                  "position_drift_vs_f64": pos, "field_drift_vs_f64": fld,
+                  "oracle_s": time.perf_counter() - t0}
+        out["checkpoints"].append(ck_out)
+        _log(f"stress: checkpoint {k}: {len(ck[0]):,} cells, bit-identical to oracle f32 {same}; "
+             f"f64 drift pos {pos}, fields {fld}")
     return out
 
 
@@ -254,7 +394,11 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cells", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=5, help="oracle steps in the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="oracle steps in the multi-core CPU baseline sample")
+    ap.add_argument("--cpu-steps-1t", type=int, default=3, help="oracle steps in the one-thread CPU sample")
+    ap.add_argument("--stress", action="store_true",
+                    help="also run the 16M-cell 512^3 stress workload (300 steps, mitotic burst) with its "
+                         "full-size oracle parity and binary64 drift checkpoints")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-sims", type=int, default=64, help="sims of the batched uneven ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
@@ -404,9 +548,10 @@ def main():
             "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic,
             "alg_bytes_per_launch": dom_bytes, "avg_launch_us": dom_avg}
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(w, args.cpu_steps)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(w, args.cpu_steps, args.cpu_steps_1t)
     ens = ensemble(ctx, args.ensemble_sims, args.ensemble_steps) if args.ensemble_sims > 0 else None
     served = serve(ctx, args.serve_rate, args.serve_seconds, args.serve_budget_gb) if args.serve_seconds > 0 else None
+    stress_out = stress(ctx) if args.stress else None
 
     out = {
         "metric": METRIC,
@@ -432,6 +577,7 @@ def main():
         "cpu_baseline": cpu,
         "ensemble": ens,
         "serving": served,
+        "stress": stress_out,
     }
     out["graph_value"] = (gcu1 - gcu0) / (g1 - g0) if gsteps else None
     print(json.dumps(out), flush=True)

@@ -52,6 +52,11 @@ struct onc_dsim {
     int key_bits;
     onc_u32 cap;
     onc_u32 seed_lo, seed_hi;
+    // This step's live count, buffer parity, completed steps and next daughter
+    // id: copied from *state by the plan kernel at the start of every step, so
+    // kernels read them together with the sim's other constants instead of
+    // one dependent load later through the state pointer.
+    onc_u32 n_live, par, step, next_id;
     float cell_radius, R, k_rep, k_adh, mobility, max_disp;
     int n_fields, n_cols;
     onc_dfield fields[ONC_DEV_MAX_FIELDS];

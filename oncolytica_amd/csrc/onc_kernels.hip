@@ -169,6 +169,11 @@ __global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
                 st->next_id = st->pend_next_id;
                 st->pending = 0u;
             }
+            onc_dsim* T = (onc_dsim*)a.sims;  // the plan is the table's only device writer
+            T[s].n_live = st->n;
+            T[s].par = st->parity;
+            T[s].step = st->step;
+            T[s].next_id = st->next_id;
             if (mode == ONC_PLAN_STEP) {
                 st->pend_n = 0u;
                 st->pend_next_id = st->next_id;
@@ -264,7 +269,7 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
             }
             const onc_dsim& S = a.sims[sa];
             const onc_u32 lo = a.cell_prefix[sa];
-            const onc_u32 par = S.state->parity;
+            const onc_u32 par = S.par;
             const float* px = (const float*)S.st[par][0];
             const float* py = (const float*)S.st[par][1];
             const float* pz = (const float*)S.st[par][2];
@@ -305,7 +310,7 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
                     if (g >= a.cell_prefix[s0 + 1]) s = onc_find_seg(a.cell_prefix, a.nsims, g);
                     const onc_dsim& S = a.sims[s];
                     const onc_u32 i = g - a.cell_prefix[s];
-                    const onc_u32 par = S.state->parity;
+                    const onc_u32 par = S.par;
                     const onc_u32 key = onc_cell_key(S, ((const float*)S.st[par][0])[i],
                                                      ((const float*)S.st[par][1])[i], ((const float*)S.st[par][2])[i]);
                     S.keys[0][i] = key;
@@ -336,7 +341,7 @@ __global__ __launch_bounds__(256) void k_hist_only(onc_kargs a) {
     for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
         const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
-        const onc_u32 n = S.state->n;
+        const onc_u32 n = S.n_live;
         const onc_u32 base = (g - a.tile_prefix[s]) * (onc_u32)(256 * ITEMS);
         for (onc_u32 i = tid; i < P * 256u; i += 256) s_hist[i] = 0u;
         __syncthreads();
@@ -390,7 +395,7 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
         const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.tile_prefix[s];
-        const onc_u32 n = S.state->n;
+        const onc_u32 n = S.state->n;  // (measured faster here than the table copy)
         const onc_u32 base = t * (onc_u32)TILE;
         const onc_u32 cnt = min((onc_u32)TILE, n - base);
         const onc_u32* kin = S.keys[pass & 1];
@@ -497,8 +502,8 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
     const onc_u32 P = a.passes;
     onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = a.sims[s];
-        const onc_u32 n = S.state->n;
-        const onc_u32 par = S.state->parity;
+        const onc_u32 n = S.n_live;
+        const onc_u32 par = S.par;
         const onc_u32* __restrict__ keys = S.keys[P & 1u];
         const onc_u32 src = S.vals[P & 1u][i];
         const onc_u32 k = keys[i];
@@ -522,7 +527,7 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
         }
         if (kprev != k) {
             S.bins[(size_t)k * 4u] = i;
-            S.bins[(size_t)k * 4u + 2u] = S.state->step + 1u;
+            S.bins[(size_t)k * 4u + 2u] = S.step + 1u;
             // uptake, once per occupied voxel, before the sweep: the bin's
             // first cell counts its run of equal keys (a few cells)
             onc_u32 cnt = 1u;
@@ -557,8 +562,8 @@ __global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
     const int tid = threadIdx.x;
     onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = a.sims[s];
-        const onc_u32 par = S.state->parity;
-        const onc_u32 epoch = S.state->step + 1u;
+        const onc_u32 par = S.par;
+        const onc_u32 epoch = S.step + 1u;
         const float4* __restrict__ p4 = (const float4*)S.pos4;
         const float4 me = p4[i];
         const float px = me.x, py = me.y, pz = me.z;
@@ -670,7 +675,7 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
     const int x0 = x4 * 4;
     const int z0 = tz * sbz;
     const int z1 = min(Z, z0 + sbz);
-    const onc_u32 par = S.state->parity;
+    const onc_u32 par = S.par;
     const float* __restrict__ in = S.fld[par][f];
     float* __restrict__ out = S.fld[par ^ 1u][f];
     const size_t XY = (size_t)X * (size_t)Y;
@@ -751,7 +756,7 @@ __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
     if (x >= X || y >= Y) return;
     const int z0 = bz * sbz;
     const int z1 = min(Z, z0 + sbz);
-    const onc_u32 par = S.state->parity;
+    const onc_u32 par = S.par;
     const float* __restrict__ in = S.fld[par][f];
     float* __restrict__ out = S.fld[par ^ 1u][f];
     const onc_dfield fd = S.fields[f];
@@ -810,7 +815,7 @@ __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
 __global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
     onc_for_cells(r.cell_prefix, r.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = r.sims[s];
-        const onc_u32 par = S.state->parity;
+        const onc_u32 par = S.par;
         onc_u32* const* T = S.st[par ^ 1u];
         onc_u32* const* C = S.st[par];
         float px = ((const float*)C[0])[i], py = ((const float*)C[1])[i], pz = ((const float*)C[2])[i];
@@ -820,7 +825,7 @@ __global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
         const onc_u32 v = onc_field_index(S, px, py, pz);
         const float o2 = S.fld[par ^ 1u][0][v];
         onc_rng rng;
-        onc_rng_init(rng, S, id, S.state->step);
+        onc_rng_init(rng, S, id, S.step);
         age = age + 1.0f;
         onc_u32 alive = 1u, div = 0u;
         float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
@@ -908,10 +913,10 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.ctile_prefix[s];
         const onc_u32 ntiles_s = a.ctile_prefix[s + 1] - a.ctile_prefix[s];
-        const onc_u32 n = S.state->n;
+        const onc_u32 n = S.n_live;
         const onc_u32 base = t * (onc_u32)TILE;
         const onc_u32 cnt = min((onc_u32)TILE, n - base);
-        const onc_u32 par = S.state->parity;
+        const onc_u32 par = S.par;
         // flags, striped load into LDS
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
@@ -992,13 +997,13 @@ __global__ __launch_bounds__(256) void k_daughters(onc_kargs a) {
         const onc_u32 nsurv = S.state->nsurv, ndiv = S.state->ndiv;
         const onc_u32 room = S.cap - nsurv;
         const onc_u32 avail = ndiv < room ? ndiv : room;
-        const onc_u32 nid = S.state->next_id;
+        const onc_u32 nid = S.next_id;
         if (k == 0u) {
             S.state->pend_n = nsurv + avail;
             S.state->pend_next_id = nid + avail;
         }
         if (k >= avail) return;
-        const onc_u32 par = S.state->parity;
+        const onc_u32 par = S.par;
         for (int c0 = 0; c0 < S.n_cols; c0 += 8) {
             onc_u32 v[8];
 #pragma unroll

@@ -1203,16 +1203,16 @@ class Translator:
         const onc_dsim& S = r.sims[s];
         RuleCtx ctx;
         ctx.S = &S;
-        ctx.par = S.state->parity;
+        ctx.par = S.par;
         ctx.i = i;
-        ctx.epoch = S.state->step + 1u;
+        ctx.epoch = S.step + 1u;
         ctx.alive = true;
         ctx.div = false;
         {cs.cname} cell = {cs.cname}_load(S, S.st[ctx.par ^ 1u], i);
         cell.pos.x = __uint_as_float(S.st[ctx.par][0][i]);
         cell.pos.y = __uint_as_float(S.st[ctx.par][1][i]);
         cell.pos.z = __uint_as_float(S.st[ctx.par][2][i]);
-        onc_rng_init(ctx.rng, S, cell._id, S.state->step);
+        onc_rng_init(ctx.rng, S, cell._id, S.step);
 {calls}
         {cs.cname}_store(S, S.st[ctx.par], i, cell);
         S.flags[i] = (ctx.alive ? 1u : 0u) | (ctx.div ? 2u : 0u);

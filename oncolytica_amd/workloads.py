@@ -153,14 +153,16 @@ def large_c2(seed: int = 42, n_cells: int = 1_000_000) -> Workload:
 
 
 def stress_c5(seed: int = 42, n_cells: int = 16_000_000, side: int = 512) -> Workload:
-    """16M cells in 32 Gaussian clusters, 512^3, O2 + glucose + drug."""
+    """16M cells in 32 Gaussian clusters (sigma 20 voxels), 512^3, three
+    fields: O2, drug, glucose (the built-in rule reads field 0 as O2 and field
+    1 as the drug; glucose diffuses and is consumed)."""
     return Workload(
         name="stress_16m_512",
         dims=(side, side, side), dx=1.0, key_format=_abi.KEY_MORTON32, capacity=2 * n_cells, seed=seed,
         fields=[FieldSpec(rate=0.1, init=1.0, bc=_abi.BC_DIRICHLET, bc_value=1.0, uptake=0.01),
-                FieldSpec(rate=0.1, init=1.0, bc=_abi.BC_DIRICHLET, bc_value=1.0, uptake=0.005),
                 FieldSpec(rate=0.05, decay=0.001, init=0.0, bc=_abi.BC_POINT_SOURCE, bc_value=1.0,
-                          src_radius=32.0)],
+                          src_radius=32.0),
+                FieldSpec(rate=0.1, init=1.0, bc=_abi.BC_DIRICHLET, bc_value=1.0, uptake=0.005)],
         cell_radius=0.4, interaction_radius=1.0, k_rep=1.0, k_adh=0.2, mobility=0.25, max_disp=0.25,
         params=[24.0, 0.3, 0.05, 0.02, 0.5], steps=300, n_cells=n_cells)
 

@@ -177,3 +177,14 @@ void oracle_morton_batch(const uint32_t* x, const uint32_t* y, const uint32_t* z
                          uint32_t* out) {
     for (uint32_t i = 0; i < n; i++) out[i] = oracle_morton(x[i], y[i], z[i], fmt);
 }
+
+/* Threads the per-cell and per-voxel loops of the oracle step use (OpenMP;
+ * results do not depend on it: every parallel loop writes disjoint outputs). */
+#ifdef _OPENMP
+#include <omp.h>
+int oracle_threads(void) { return omp_get_max_threads(); }
+void oracle_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+#else
+int oracle_threads(void) { return 1; }
+void oracle_set_threads(int n) { (void)n; }
+#endif

@@ -71,6 +71,7 @@ uint32_t FN(oracle_voxel_key)(REAL px, REAL py, REAL pz, REAL dx, const int32_t 
 
 void FN(oracle_keys)(const REAL* px, const REAL* py, const REAL* pz, uint32_t n, const int32_t dims[3],
                      REAL dx, int fmt, uint32_t* keys) {
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < n; i++) keys[i] = FN(oracle_voxel_key)(px[i], py[i], pz[i], dx, dims, fmt);
 }
 
@@ -189,6 +190,7 @@ void FN(oracle_field_init)(const onc_field_desc* f, const int32_t d[3], REAL* u)
 void FN(oracle_stencil7)(const REAL* in, REAL* out, const int32_t d[3], const onc_field_desc* f) {
     const size_t X = (size_t)d[0], XY = (size_t)d[0] * (size_t)d[1];
     REAL rate = (REAL)f->rate, decay = (REAL)f->decay, lo = (REAL)f->lo, hi = (REAL)f->hi;
+#pragma omp parallel for schedule(static)
     for (int32_t z = 0; z < d[2]; z++)
         for (int32_t y = 0; y < d[1]; y++)
             for (int32_t x = 0; x < d[0]; x++) {
@@ -395,6 +397,7 @@ int FN(oracle_tumor_run)(const onc_sim_desc* d, uint32_t* n_io, uint32_t* next_i
         FN(oracle_keys)(px, py, pz, n, d->dims, dx, d->key_format, keys);
         for (uint32_t i = 0; i < n; i++) perm[i] = i;
         oracle_radix_sort_pairs(keys, perm, n, kb);
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < n; i++) {
             uint32_t j = perm[i];
             t[0][i] = px[j]; t[1][i] = py[j]; t[2][i] = pz[j]; t[3][i] = age[j];
@@ -416,6 +419,7 @@ int FN(oracle_tumor_run)(const onc_sim_desc* d, uint32_t* n_io, uint32_t* next_i
             i = e;
         }
         /* 5: forces and integration into px/py/pz (old order no longer needed) */
+#pragma omp parallel for schedule(dynamic, 4096)
         for (uint32_t i = 0; i < n; i++) {
             REAL Fx, Fy, Fz;
             force_on(d, keys, n, t[0], t[1], t[2], i, &Fx, &Fy, &Fz);
@@ -427,6 +431,7 @@ int FN(oracle_tumor_run)(const onc_sim_desc* d, uint32_t* n_io, uint32_t* next_i
         for (int f = 0; f < d->n_fields; f++) FN(oracle_stencil7)(fields[f], swept[f], d->dims, &d->fields[f]);
         for (int f = 0; f < d->n_fields; f++) memcpy(fields[f], swept[f], nvox * sizeof(REAL));
         /* 7: rule */
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < n; i++) {
             REAL a = t[3][i];
             uint32_t g = tgen[i];

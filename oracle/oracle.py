@@ -43,6 +43,9 @@ def lib() -> C.CDLL:
         L.oracle_exscan_u32.argtypes = [P, P, u32]
         L.oracle_bins.argtypes = [P, u32, P, P]
         L.oracle_philox_batch.argtypes = [P, u32, C.c_uint64, P]
+        L.oracle_threads.restype = C.c_int
+        L.oracle_threads.argtypes = []
+        L.oracle_set_threads.argtypes = [C.c_int]
         for sfx, real in (("f32", C.c_float), ("f64", C.c_double)):
             getattr(L, f"oracle_keys_{sfx}").argtypes = [P, P, P, u32, P, real, C.c_int, P]
             getattr(L, f"oracle_sincos_q_{sfx}").argtypes = [real, P, P]
@@ -193,3 +196,12 @@ class TumorState:
         n = self.n.value
         return [self.px[:n].copy(), self.py[:n].copy(), self.pz[:n].copy(), self.id[:n].copy(),
                 self.age[:n].copy(), self.gen[:n].copy()]
+
+
+def threads() -> int:
+    """Threads the oracle's per-cell and per-voxel loops use (OpenMP)."""
+    return int(lib().oracle_threads())
+
+
+def set_threads(n: int) -> None:
+    lib().oracle_set_threads(int(n))

@@ -356,3 +356,23 @@ def test_tumor_c1_f32_close_to_f64_over_10_steps():
     # its uptake; everywhere else the fields agree to binary32 rounding
     dfield = np.abs(a.fields[0] - b.fields[0])
     assert np.mean(dfield > 1e-5) < 0.01 and dfield.max() < 0.02
+
+
+def test_step_identical_for_any_thread_count():
+    """The oracle's OpenMP loops write disjoint outputs: one thread and many
+    give the same bits."""
+    w = W.large_c2(n_cells=20_000)
+    d = w.desc()
+    cols = W.initial_cells(w)
+    n_threads = O.threads()
+    out = []
+    try:
+        for t in (1, max(2, n_threads)):
+            O.set_threads(t)
+            st = O.TumorState(d, cols)
+            st.run(3)
+            out.append((st.columns(), [f.copy() for f in st.fields]))
+    finally:
+        O.set_threads(n_threads)
+    for a, b in zip(out[0][0] + out[0][1], out[1][0] + out[1][1]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
