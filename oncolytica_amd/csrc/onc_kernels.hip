@@ -556,9 +556,77 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // ---------------------------------------------------------------------------
 #define ONC_PAIR_LIST 16
 
+// Candidates of the 27 Moore bins around (px, py, pz), z-plane by z-plane, in
+// template.wgsl:11-21 order (x fastest) and bin order inside a bin, skipping
+// `self`; `pair(o, j)` gets each candidate's packed position. Per plane the
+// nine bin entries are loaded together, the non-empty ranges are compacted
+// into the thread's LDS slots, and a cursor walks them four candidates at a
+// time (the four position loads are in flight together); the cursor only
+// reads LDS when it moves to the next range.
+template <class F>
+__device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
+                                            onc_u32 self, const float4* __restrict__ pos4, onc_u32 (*s_rs)[256],
+                                            onc_u32 (*s_re)[256], F pair) {
+    const int tid = threadIdx.x;
+    const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
+              vz = onc_vox1(pz, S.dx, S.dims[2]);
+    const uint4* __restrict__ bins = (const uint4*)S.bins;
+    for (int oz = -1; oz <= 1; oz++) {
+        const int nz = vz + oz;
+        if (nz < 0 || nz >= S.dims[2]) continue;
+        uint4 b[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const int ny = vy + q / 3 - 1, nx = vx + q % 3 - 1;
+            const bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
+            b[q] = make_uint4(0u, 0u, 0u, 0u);
+            if (in) b[q] = bins[onc_morton((onc_u32)nx, (onc_u32)ny, (onc_u32)nz, S.key_format)];
+        }
+        onc_u32 nr = 0;
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            if (b[q].z == epoch && b[q].y > b[q].x) {
+                s_rs[nr][tid] = b[q].x;
+                s_re[nr][tid] = b[q].y;
+                nr++;
+            }
+        }
+        onc_u32 k = 0, j = 0, e = 0;
+        if (nr > 0) {
+            j = s_rs[0][tid];
+            e = s_re[0][tid];
+        }
+        while (k < nr) {
+            onc_u32 jj[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                jj[u] = k < nr ? j : 0xFFFFFFFFu;
+                if (k < nr) {
+                    j++;
+                    if (j == e) {
+                        k++;
+                        if (k < nr) {
+                            j = s_rs[k][tid];
+                            e = s_re[k][tid];
+                        }
+                    }
+                }
+            }
+            float4 p[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) p[u] = jj[u] != 0xFFFFFFFFu ? pos4[jj[u]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (jj[u] != 0xFFFFFFFFu && jj[u] != self) pair(p[u], jj[u]);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
     // per-thread list of interacting neighbours, [slot][thread] (bank-conflict free)
     __shared__ onc_u32 s_pairs[ONC_PAIR_LIST][256];
+    // per-thread non-empty bin ranges of the current z-plane
+    __shared__ onc_u32 s_rs[9][256], s_re[9][256];
     const int tid = threadIdx.x;
     onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = a.sims[s];
@@ -572,33 +640,43 @@ __global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
         float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
         // Phase B: the pair law over the listed neighbours, in visiting order.
         onc_u32 npair = 0;
-        auto flush = [&]() {
-            for (onc_u32 k = 0; k < npair; k++) {
-                const float4 o = p4[s_pairs[k][tid]];
-                const float ddx = px - o.x;
-                const float ddy = py - o.y;
-                const float ddz = pz - o.z;
-                float d2 = ddx * ddx + ddy * ddy;
-                d2 = d2 + ddz * ddz;
-                const float dist = __builtin_sqrtf(d2);
-                float f;
-                if (dist < Sc) {
-                    f = krep * (Sc - dist);
-                } else {
-                    if (kadh == 0.0f) continue;
-                    f = -(kadh * ((R - dist) / RmS));
-                }
-                const float sc = f / dist;
-                Fx = Fx + sc * ddx;
-                Fy = Fy + sc * ddy;
-                Fz = Fz + sc * ddz;
+        auto law = [&](const float4& o) {
+            const float ddx = px - o.x;
+            const float ddy = py - o.y;
+            const float ddz = pz - o.z;
+            float d2 = ddx * ddx + ddy * ddy;
+            d2 = d2 + ddz * ddz;
+            const float dist = __builtin_sqrtf(d2);
+            float f;
+            if (dist < Sc) {
+                f = krep * (Sc - dist);
+            } else {
+                if (kadh == 0.0f) return;
+                f = -(kadh * ((R - dist) / RmS));
             }
+            const float sc = f / dist;
+            Fx = Fx + sc * ddx;
+            Fy = Fy + sc * ddy;
+            Fz = Fz + sc * ddz;
+        };
+        // the listed neighbours' positions are loaded four at a time, then
+        // accumulated in list order
+        auto flush = [&]() {
+            onc_u32 k = 0;
+            for (; k + 4u <= npair; k += 4u) {
+                float4 o[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) o[u] = p4[s_pairs[k + u][tid]];
+#pragma unroll
+                for (int u = 0; u < 4; u++) law(o[u]);
+            }
+            for (; k < npair; k++) law(p4[s_pairs[k][tid]]);
             npair = 0;
         };
         // Phase A: cheap distance filter over the 27 bins; only pairs inside
         // the interaction radius reach the expensive (sqrt, divide) path, so
         // lanes diverge on short lists instead of on every candidate.
-        onc_for_neighbor_pos4(S, epoch, px, py, pz, i, p4, [&](const float4& o, onc_u32 j) {
+        forces_walk(S, epoch, px, py, pz, i, p4, s_rs, s_re, [&](const float4& o, onc_u32 j) {
             const float ddx = px - o.x;
             const float ddy = py - o.y;
             const float ddz = pz - o.z;
