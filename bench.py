@@ -181,7 +181,8 @@ def serve(ctx, rate: float, seconds: float, budget_gb: float):
     out = rep.summary()
     out.update({"workload": "served_poisson", "rate_jobs_per_s": rate, "arrival_window_s": seconds})
     _log(f"serving: p50 {out['p50_latency_s']:.2f} s p99 {out['p99_latency_s']:.2f} s, "
-         f"{out['cell_updates_per_s'] / 1e6:.0f} M cell-updates/s, peak {out['peak_resident_jobs']} resident")
+         f"{out['cell_updates_per_s'] / 1e6:.0f} M cell-updates/s, peak {out['peak_resident_jobs']} resident, "
+         f"{out['ticks']} ticks, host {out['host_times_s']}")
     return out
 
 

@@ -168,6 +168,11 @@ int onc_sim_step(onc_sim* sim, int nsteps);
 int onc_sim_sync(onc_sim* sim);
 /* synchronises; returns live cells, next id and completed steps */
 int onc_sim_counts(onc_sim* sim, uint32_t* n, uint32_t* next_id, uint32_t* steps);
+/* The sim advances only while its completed steps are below `stop`; later
+   steps of its batch leave it untouched (no reference counterpart: it lets a
+   batch hold jobs with different horizons without host round trips).
+   Default: no limit (0xFFFFFFFF). */
+int onc_sim_set_stop(onc_sim* sim, uint32_t stop);
 /* sum over completed steps of the live cells at the start of each step */
 int onc_sim_cell_updates(onc_sim* sim, uint64_t* cell_updates);
 /* device bytes this sim holds (HBM admission accounting) */

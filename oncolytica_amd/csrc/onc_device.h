@@ -38,7 +38,7 @@ struct onc_dstate {
     onc_u32 nsurv, ndiv;           // written by the compaction scan
     onc_u32 pend_n, pend_next_id;  // written by the daughter append
     onc_u32 pending;               // a step is in flight, commit before use
-    onc_u32 pad0;
+    onc_u32 stop;                  // the sim advances only while step < stop
     onc_u64 cell_updates;          // sum over steps of live cells at step start
     onc_u32 pad[4];
 };
@@ -57,6 +57,8 @@ struct onc_dsim {
     // kernels read them together with the sim's other constants instead of
     // one dependent load later through the state pointer.
     onc_u32 n_live, par, step, next_id;
+    onc_u32 frozen;  // step >= stop: this step leaves the sim untouched
+    onc_u32 pad_[3];
     float cell_radius, R, k_rep, k_adh, mobility, max_disp;
     int n_fields, n_cols;
     onc_dfield fields[ONC_DEV_MAX_FIELDS];

@@ -169,12 +169,15 @@ __global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
                 st->next_id = st->pend_next_id;
                 st->pending = 0u;
             }
+            // a sim at its stop step takes part in no kernel of this step
+            const bool frozen = st->step >= st->stop;
             onc_dsim* T = (onc_dsim*)a.sims;  // the plan is the table's only device writer
-            T[s].n_live = st->n;
+            T[s].n_live = frozen ? 0u : st->n;
             T[s].par = st->parity;
             T[s].step = st->step;
             T[s].next_id = st->next_id;
-            if (mode == ONC_PLAN_STEP) {
+            T[s].frozen = frozen ? 1u : 0u;
+            if (mode == ONC_PLAN_STEP && !frozen) {
                 st->pend_n = 0u;
                 st->pend_next_id = st->next_id;
                 st->nsurv = 0u;
@@ -560,8 +563,8 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // template.wgsl:11-21 order (x fastest) and bin order inside a bin, skipping
 // `self`; `pair(o, j)` gets each candidate's packed position. Per plane the
 // nine bin entries are loaded together, the non-empty ranges are compacted
-// into the thread's LDS slots, and a cursor walks them four candidates at a
-// time (the four position loads are in flight together); the cursor only
+// into the thread's LDS slots, and a cursor walks them eight candidates at a
+// time (the eight position loads are in flight together); the cursor only
 // reads LDS when it moves to the next range.
 template <class F>
 __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
@@ -597,9 +600,9 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, fl
             e = s_re[0][tid];
         }
         while (k < nr) {
-            onc_u32 jj[4];
+            onc_u32 jj[8];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < 8; u++) {
                 jj[u] = k < nr ? j : 0xFFFFFFFFu;
                 if (k < nr) {
                     j++;
@@ -612,11 +615,11 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, fl
                     }
                 }
             }
-            float4 p[4];
+            float4 p[8];
 #pragma unroll
-            for (int u = 0; u < 4; u++) p[u] = jj[u] != 0xFFFFFFFFu ? pos4[jj[u]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (int u = 0; u < 8; u++) p[u] = jj[u] != 0xFFFFFFFFu ? pos4[jj[u]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < 8; u++)
                 if (jj[u] != 0xFFFFFFFFu && jj[u] != self) pair(p[u], jj[u]);
         }
     }
@@ -815,6 +818,7 @@ __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
     const int jb = (int)onc_uniform((onc_u32)onc_find_seg(a.jobs_prefix, a.njobs, b));
     const onc_u32 job = onc_uniform(a.jobs[jb]);
     const onc_dsim& S = a.sims[job >> 8];
+    if (S.frozen) return;
     const int f = (int)(job & 255u);
     const int X = S.dims[0], Y = S.dims[1], Z = S.dims[2];
     const int sbx = a.sbx, sby = a.sby, sbz = a.sbz;

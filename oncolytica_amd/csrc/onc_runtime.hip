@@ -426,6 +426,7 @@ extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) 
     // initial device state
     hipStream_t st = c->stream;
     onc_dstate z{};
+    z.stop = 0xFFFFFFFFu;
     hipError_t e = hipMemcpyAsync(s->dstate, &z, sizeof(z), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = onc_launch_fill_u32(s->dsim.bins, 0xFFFFFFFFu, 4ull * s->table, st);
     for (int f = 0; f < d->n_fields && e == hipSuccess; f++) {
@@ -544,6 +545,16 @@ extern "C" int onc_sim_download_field(onc_sim* s, int f, float* host) {
     onc_dstate st;
     CKR(sim_read_state(s, &st));
     CK(hipMemcpy(host, s->dsim.fld[st.parity][f], s->nvox * 4, hipMemcpyDeviceToHost));
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_set_stop(onc_sim* s, uint32_t stop) {
+    if (!s) return fail(ONC_ERR_INVALID, "NULL sim");
+    CK(hipSetDevice(s->ctx->device));
+    onc_dstate st;
+    CKR(sim_read_state(s, &st));
+    st.stop = stop;
+    CK(hipMemcpy(s->dstate, &st, sizeof(st), hipMemcpyHostToDevice));
     return ONC_OK;
 }
 
@@ -1070,6 +1081,7 @@ extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* v
     ds.state = (onc_dstate*)(B + o_st);
     ds.cap = n;
     onc_dstate dst{};
+    dst.stop = 0xFFFFFFFFu;
     dst.n = n;
     hipStream_t st = c->stream;
     CK(hipMemcpyAsync(B + o_sim, &ds, sizeof(ds), hipMemcpyHostToDevice, st));
@@ -1165,6 +1177,7 @@ extern "C" int onc_stencil7(onc_ctx* c, const float* in, float* out, const int32
     char* B = (char*)base;
     ds.state = (onc_dstate*)(B + o_st);
     onc_dstate dst{};
+    dst.stop = 0xFFFFFFFFu;
     const onc_u32 nb = stencil_blocks(dims[0], dims[1], dims[2], sbx, sby, sbz);
     onc_u32 jobs[4] = {0u, 0u, nb, 0u};  // jobs[0] = job 0, jobs_prefix = {0, nb}
     CK(hipMemcpyAsync(B + o_sim, &ds, sizeof(ds), hipMemcpyHostToDevice, c->stream));

@@ -106,6 +106,11 @@ class Sim:
         call("onc_sim_download_field", self.h, f, ptr(out))
         return out
 
+    def set_stop(self, stop: int) -> None:
+        """Advance only while completed steps < stop (later batch steps leave
+        the sim untouched)."""
+        call("onc_sim_set_stop", self.h, int(stop) & 0xFFFFFFFF)
+
     def step(self, n: int = 1) -> None:
         call("onc_sim_step", self.h, n)
 

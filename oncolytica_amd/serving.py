@@ -6,12 +6,15 @@ device footprint (`onc_sim_footprint`: cell state, sort buffers, fields and
 bin table) fits in the HBM budget and a slot is free. Admitted jobs stay
 resident and are folded into one batch (`onc_batch_*`: one launch per phase
 over all resident sims, one captured hipGraph per timestep replayed for the
-whole tick). A tick advances every resident job by the same number of steps,
-k = min(steps left of any resident job, max_tick_steps), so every job ends
-exactly at its horizon. Finished jobs leave the batch and free their memory,
-and jobs that arrived meanwhile join at the next tick. While the GPU runs a
-tick, the host builds the initial state of the jobs that will be admitted
-next.
+whole tick). Each job's horizon is set on the device (`onc_sim_set_stop`):
+a job that reaches it is left untouched by the rest of the tick, so a tick
+can run k = min(max_tick_steps, most steps left of any resident job) steps
+with no host round trip, and every job still ends exactly at its horizon.
+Jobs that finished during a tick leave the batch at its end and free their
+memory; jobs that arrived meanwhile join at the next tick. While the GPU runs
+a tick, the host builds the initial state of the jobs that will be admitted
+next. Jobs admitted during a tick are resident (allocated, uploaded) while it
+runs and join the batch when it ends.
 
 A job's result does not depend on what else is in the batch: a batched step
 is bit-identical to stepping the sim alone.
@@ -60,6 +63,7 @@ class ServeReport:
     peak_resident: int
     peak_bytes: int
     budget_bytes: int
+    host_times: dict = field(default_factory=dict)
 
     def summary(self) -> dict:
         lat = np.array([j.latency for j in self.jobs if not math.isnan(j.finished)])
@@ -78,6 +82,7 @@ class ServeReport:
             "peak_resident_jobs": self.peak_resident,
             "peak_resident_bytes": self.peak_bytes,
             "hbm_budget_bytes": self.budget_bytes,
+            "host_times_s": self.host_times,
         }
 
 
@@ -137,56 +142,78 @@ class Server:
             self.tuning.apply(batch)
         pending = list(jobs)       # not yet arrived
         waiting: list[Job] = []    # arrived, not admitted
-        active: list[Job] = []
+        staged: list[Job] = []     # admitted: resident on the device, joining the batch at the next tick
+        active: list[Job] = []     # in the batch
         used = 0
         ticks = graph_ticks = peak_res = peak_bytes = 0
         total_cu = 0
+        times = {"admit_s": 0.0, "retire_s": 0.0, "launch_s": 0.0, "gpu_wait_s": 0.0}
         t0 = clock()
+
+        def arrivals():
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.pop(0))
+
+        def admit():
+            # arrival order, first fit within the budget and the slots
+            nonlocal used
+            ta = clock()
+            for j in list(waiting):
+                if len(active) + len(staged) >= self.max_jobs:
+                    break
+                if used + j.footprint > self.budget:
+                    continue
+                self._prepare(j)
+                j.sim = Sim(self.ctx, j.workload.desc())
+                j.sim.upload_cells(j.cols0)
+                j.sim.set_stop(j.horizon)
+                j.cols0 = None
+                j.left = j.horizon
+                j.admitted = clock() - t0
+                used += j.footprint
+                waiting.remove(j)
+                staged.append(j)
+            times["admit_s"] += clock() - ta
+
         try:
-            while pending or waiting or active:
-                now = clock() - t0
-                while pending and pending[0].arrival <= now:
-                    waiting.append(pending.pop(0))
-                # admission: arrival order, first fit within budget and slots
-                admitted_any = False
-                for j in list(waiting):
-                    if len(active) >= self.max_jobs:
-                        break
-                    if used + j.footprint > self.budget:
-                        continue
-                    self._prepare(j)
-                    j.sim = Sim(self.ctx, j.workload.desc())
-                    j.sim.upload_cells(j.cols0)
-                    j.cols0 = None
+            while pending or waiting or staged or active:
+                arrivals()
+                for j in staged:
                     batch.add(j.sim)
-                    j.left = j.horizon
-                    j.admitted = clock() - t0
-                    used += j.footprint
-                    waiting.remove(j)
                     active.append(j)
-                    admitted_any = True
-                if admitted_any and self.log:
+                if staged and self.log:
                     self.log(f"t={clock() - t0:7.2f}s resident={len(active)} waiting={len(waiting)} "
                              f"used={used / 2**30:.1f} GiB")
-                peak_res = max(peak_res, len(active))
-                peak_bytes = max(peak_bytes, used)
+                staged.clear()
                 if not active:
-                    if pending:
+                    admit()
+                    if not staged and pending:
                         sleep(max(0.0, min(0.05, pending[0].arrival - (clock() - t0))))
                     continue
-                k = min(min(j.left for j in active), self.max_tick_steps)
+                peak_res = max(peak_res, len(active))
+                peak_bytes = max(peak_bytes, used)
+                k = min(max(j.left for j in active), self.max_tick_steps)
                 use_graph = k >= self.graph_min_steps
+                tl = clock()
                 batch.step(k, graph=use_graph)
+                times["launch_s"] += clock() - tl
                 ticks += 1
                 graph_ticks += int(use_graph)
-                # host work overlapping the tick: initial state of the next arrivals
+                # host work overlapping the tick: admit arrived jobs (device
+                # allocation, upload) and build the initial state of the next
+                arrivals()
+                admit()
                 horizon_t = clock() - t0
                 for j in waiting + [p for p in pending[:4] if p.arrival <= horizon_t + 0.5]:
                     self._prepare(j)
+                tw = clock()
                 batch.sync()
+                times["gpu_wait_s"] += clock() - tw
                 done_t = clock() - t0
+                tr = clock()
                 for j in list(active):
-                    j.left -= k
+                    j.left = max(0, j.left - k)
                     if j.left == 0:
                         j.finished = done_t
                         j.cell_updates = _cu(j.sim)
@@ -199,14 +226,16 @@ class Server:
                         j.sim = None
                         used -= j.footprint
                         active.remove(j)
+                times["retire_s"] += clock() - tr
         finally:
-            for j in active:
+            for j in active + staged:
                 if j.sim is not None:
                     j.sim.close()
             batch.close()
         wall = clock() - t0
         return ServeReport(jobs=jobs, wall_s=wall, cell_updates=total_cu, ticks=ticks, graph_ticks=graph_ticks,
-                           peak_resident=peak_res, peak_bytes=peak_bytes, budget_bytes=self.budget)
+                           peak_resident=peak_res, peak_bytes=peak_bytes, budget_bytes=self.budget,
+                           host_times=times)
 
 
 def _cu(sim: Sim) -> int:

@@ -117,3 +117,33 @@ def test_oversized_job_rejected(ctx):
     j = _jobs(1)[0]
     with pytest.raises(ValueError, match="budget"):
         Server(ctx, hbm_budget=1024).run([j])
+
+
+def test_stop_step_freezes_a_sim_in_its_batch(ctx):
+    """A sim whose completed steps reach its stop is left untouched by later
+    batch steps (eager and graph); its state equals that many steps alone."""
+    ws = W.ensemble_c3(n_sims=3, scale=0.02)
+    stops = [5, 12, 1000]
+    alone = []
+    for w, k in zip(ws, stops):
+        s = Sim(ctx, w.desc())
+        s.upload_cells(W.initial_cells(w))
+        s.step(min(k, 15))
+        s.sync()
+        alone.append((s.download_cells(), [s.download_field(0)]))
+        s.close()
+    sims = [Sim(ctx, w.desc()) for w in ws]
+    b = Batch(ctx)
+    for s, w, k in zip(sims, ws, stops):
+        s.upload_cells(W.initial_cells(w))
+        s.set_stop(k)
+        b.add(s)
+    b.step(8, graph=True)
+    b.step(7, graph=False)
+    b.sync()
+    for s, k, (cols, fields) in zip(sims, stops, alone):
+        assert s.counts()[2] == min(k, 15)
+        _same(cols, fields, s.download_cells(), [s.download_field(0)])
+    b.close()
+    for s in sims:
+        s.close()

@@ -33,6 +33,9 @@ class FakeSim:
     def upload_cells(self, cols):
         pass
 
+    def set_stop(self, stop):
+        self.stop = stop
+
     def close(self):
         FakeSim.live -= 1
 
@@ -52,7 +55,7 @@ class FakeBatch:
 
     def step(self, k, graph=True):
         for s in self.sims:
-            s.steps += k
+            s.steps = min(s.steps + k, s.stop)
         self.log.append((k, len(self.sims), graph))
         self.clock.t += k * self.cost
 
@@ -113,12 +116,16 @@ def test_budget_and_first_fit_admission(fake):
     assert [j.cell_updates for j in jobs] == [50, 10, 10]
 
 
-def test_graph_only_for_long_ticks(fake):
+def test_ticks_run_to_the_longest_horizon_and_graph_only_when_long(fake):
     clock, batches = fake
     jobs = _jobs([(0.0, 10, 2), (0.0, 10, 40)])
+    serving.Server(None, hbm_budget=10**9, graph_min_steps=4, max_tick_steps=16).run(
+        jobs, clock=clock, sleep=clock.sleep)
+    assert batches[0].log[0] == (16, 2, True)   # the short job stops on the device after 2 steps
+    assert [j.cell_updates for j in jobs] == [2, 40]
+    jobs = _jobs([(0.0, 10, 2), (0.0, 10, 3)])
     serving.Server(None, hbm_budget=10**9, graph_min_steps=4).run(jobs, clock=clock, sleep=clock.sleep)
-    assert batches[0].log[0] == (2, 2, False)
-    assert batches[0].log[1][2] is True
+    assert batches[1].log == [(3, 2, False)]
 
 
 def test_rejects_oversized_and_empty_horizon(fake):
