@@ -17,6 +17,7 @@ typedef unsigned long long onc_u64;
 #define ONC_DEV_MAX_COLS 48
 #define ONC_DEV_MAX_PARAMS 32
 #define ONC_DEV_MAX_PASSES 4
+#define ONC_HIST_COPIES 8  // copies of the per-sim digit histograms (atomic spread)
 
 #define ONC_FLAG_ALIVE 1u
 #define ONC_FLAG_DIV 2u
@@ -90,7 +91,7 @@ struct onc_kargs {
     onc_u32 max_ctiles;   // compaction tiles the status array holds
     onc_u32* ctile_prefix;// [nsims + 1] compaction tile prefix of live counts
     onc_u32* cell_prefix; // [nsims + 1] prefix of live counts
-    onc_u32* hist;        // [nsims][4][256] digit histograms
+    onc_u32* hist;        // [ONC_HIST_COPIES][nsims][4][256] digit histograms
     onc_u32* counters;    // [16] dynamic tile counters
     onc_u32* sort_status; // [4][max_tiles][256] look-back words
     onc_u64* comp_status; // [max_ctiles] look-back words of the compaction scan

@@ -145,7 +145,9 @@ __device__ __forceinline__ void lookback_wave64(const onc_u64* status, onc_u32 g
 
 // ---------------------------------------------------------------------------
 // plan / commit (one workgroup): commits the previous step's counters, plans
-// this step's tiles from the live counts, clears histograms and tile counters.
+// this step's tiles from the live counts, clears the tile counters (the
+// digit histograms are cleared by the permute kernel, once the sort passes
+// have read them).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
     __shared__ onc_u32 s_tmp[8];
@@ -214,7 +216,6 @@ __global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
             a.cell_prefix[a.nsims] = s_carry_c;
             a.ctile_prefix[a.nsims] = s_carry_ct;
         }
-        for (onc_u32 i = tid; i < a.nsims * 4u * 256u; i += 256) a.hist[i] = 0u;
         for (int i = tid; i < ONC_NCOUNTERS; i += 256) a.counters[i] = 0u;
     }
 }
@@ -240,11 +241,14 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
     const onc_u32 c1 = (onc_u32)(((unsigned long long)nch * (blk + 1u)) / gridDim.x);
     for (onc_u32 i = tid; i < P * 256u; i += 256) s_hist[i] = 0u;
     __syncthreads();
+    // this block's copy of the digit histograms (blocks spread their device
+    // atomics over ONC_HIST_COPIES copies; the sort passes add the copies)
+    onc_u32* hist_copy = a.hist + (size_t)(blockIdx.x % ONC_HIST_COPIES) * a.nsims * 4u * 256u;
     auto flush = [&](int s) {
         __syncthreads();
         for (onc_u32 i = tid; i < P * 256u; i += 256) {
             const onc_u32 v = s_hist[i];
-            if (v) atomicAdd(&a.hist[(size_t)s * 4u * 256u + i], v);
+            if (v) atomicAdd(&hist_copy[(size_t)s * 4u * 256u + i], v);
             s_hist[i] = 0u;
         }
         __syncthreads();
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
             for (onc_u32 p = 0; p < P; p++) atomicAdd(&s_hist[p * 256u + ((key >> (8u * p)) & 255u)], 1u);
         } else {
             for (onc_u32 p = 0; p < P; p++)
-                atomicAdd(&a.hist[((size_t)s * 4u + p) * 256u + ((key >> (8u * p)) & 255u)], 1u);
+                atomicAdd(&hist_copy[((size_t)s * 4u + p) * 256u + ((key >> (8u * p)) & 255u)], 1u);
         }
     };
     int cur = -1;
@@ -455,7 +459,10 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
             prefix = lookback_digit(status, g, g - t, (onc_u32)tid, a.err);
             st_relaxed(mine, ONC_FLAG_P | (prefix + tot));
         }
-        const onc_u32 hd = a.hist[((size_t)s * 4u + (onc_u32)pass) * 256u + tid];
+        onc_u32 hd = 0u;
+#pragma unroll
+        for (int c = 0; c < ONC_HIST_COPIES; c++)
+            hd += a.hist[(((size_t)c * a.nsims + (size_t)s) * 4u + (onc_u32)pass) * 256u + tid];
         const onc_u32 gex = block_exscan256(hd, s_tmp, nullptr);
         s_gbase[tid] = gex + prefix;
         __syncthreads();
@@ -503,6 +510,11 @@ __device__ __forceinline__ bool onc_is_fixed(const onc_dfield& f, const int* d, 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
     const onc_u32 P = a.passes;
+    // the sort passes are done with this step's digit histograms: clear them
+    // for the next step's Morton kernel
+    for (onc_u32 i = blockIdx.x * 256u + threadIdx.x; i < ONC_HIST_COPIES * a.nsims * 4u * 256u;
+         i += gridDim.x * 256u)
+        a.hist[i] = 0u;
     onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = a.sims[s];
         const onc_u32 n = S.n_live;

@@ -671,8 +671,10 @@ static int batch_rebuild(onc_batch* b) {
         b->d_sims = (onc_dsim*)p;
         if (hipMalloc(&p, (3 * (cs + 1) + ONC_NCOUNTERS + 16) * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "batch small");
         b->d_small = (onc_u32*)p;
-        if (hipMalloc(&p, cs * 4 * 256 * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "batch hist");
+        if (hipMalloc(&p, (size_t)ONC_HIST_COPIES * cs * 4 * 256 * 4) != hipSuccess)
+            return fail(ONC_ERR_NOMEM, "batch hist");
         b->d_hist = (onc_u32*)p;
+        CK(hipMemsetAsync(b->d_hist, 0, (size_t)ONC_HIST_COPIES * cs * 4 * 256 * 4, b->stream));
         if (hipMalloc(&p, (size_t)ONC_DEV_MAX_PASSES * ct * 256 * 4) != hipSuccess) return fail(ONC_ERR_NOMEM, "sort status");
         b->d_sort_status = (onc_u32*)p;
         if (hipMalloc(&p, cct * 8) != hipSuccess) return fail(ONC_ERR_NOMEM, "compaction status");
@@ -1067,7 +1069,7 @@ extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* v
     size_t o_k0 = take(n * 4ull), o_k1 = take(n * 4ull), o_v0 = take(n * 4ull), o_v1 = take(n * 4ull);
     size_t o_sim = take(sizeof(onc_dsim)), o_st = take(sizeof(onc_dstate));
     size_t o_small = take((3 * 2 + ONC_NCOUNTERS + 16) * 4);
-    size_t o_hist = take(4 * 256 * 4);
+    size_t o_hist = take((size_t)ONC_HIST_COPIES * 4 * 256 * 4);
     size_t o_ss = take((size_t)ONC_DEV_MAX_PASSES * tiles * 256 * 4);
     size_t o_cs = take(tiles * 8);
     void* base;
@@ -1087,6 +1089,7 @@ extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* v
     CK(hipMemcpyAsync(B + o_sim, &ds, sizeof(ds), hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(B + o_st, &dst, sizeof(dst), hipMemcpyHostToDevice, st));
     CK(hipMemsetAsync(B + o_small, 0, (3 * 2 + ONC_NCOUNTERS + 16) * 4, st));
+    CK(hipMemsetAsync(B + o_hist, 0, (size_t)ONC_HIST_COPIES * 4 * 256 * 4, st));
     CK(hipMemcpyAsync(ds.keys[0], kin, n * 4ull, hipMemcpyDeviceToDevice, st));
     CK(hipMemcpyAsync(ds.vals[0], vin, n * 4ull, hipMemcpyDeviceToDevice, st));
     onc_kargs a{};
