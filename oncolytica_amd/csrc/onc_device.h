@@ -19,6 +19,7 @@ typedef unsigned long long onc_u64;
 #define ONC_DEV_MAX_PASSES 4
 #define ONC_HIST_COPIES 8  // copies of the per-sim digit histograms (atomic spread)
 
+#define ONC_COL_ID_DEV 3  // the id column of every cell schema
 #define ONC_FLAG_ALIVE 1u
 #define ONC_FLAG_DIV 2u
 
@@ -30,15 +31,19 @@ struct onc_dfield {
 };
 
 // Dynamic per-sim counters, updated on the device only (graph replays need
-// no host round trip).
+// no host round trip). A step is committed by the next plan (or before the
+// host reads the sim): the live cells are then st[parity][c][0, split)
+// (survivors, in sorted order) followed by dau2[c][0, n - split) (the step's
+// daughters in parent order).
 struct onc_dstate {
     onc_u32 n;        // live cells
     onc_u32 next_id;  // id of the next daughter
     onc_u32 step;     // completed steps (Philox counter word 1)
     onc_u32 parity;   // which state / field buffer is current
     onc_u32 nsurv, ndiv;           // written by the compaction scan
-    onc_u32 pend_n, pend_next_id;  // written by the daughter append
-    onc_u32 pending;               // a step is in flight, commit before use
+    onc_u32 split;                 // live cells held in st[parity]; the rest are in dau2
+    onc_u32 pending;               // a step ran and is not committed yet
+    onc_u32 pad2;
     onc_u32 stop;                  // the sim advances only while step < stop
     onc_u64 cell_updates;          // sum over steps of live cells at step start
     onc_u32 pad[4];
@@ -59,7 +64,8 @@ struct onc_dsim {
     // one dependent load later through the state pointer.
     onc_u32 n_live, par, step, next_id;
     onc_u32 frozen;  // step >= stop: this step leaves the sim untouched
-    onc_u32 pad_[3];
+    onc_u32 split;   // cells [0, split) are in st[par], [split, n_live) in dau2
+    onc_u32 pad_[2];
     float cell_radius, R, k_rep, k_adh, mobility, max_disp;
     int n_fields, n_cols;
     onc_dfield fields[ONC_DEV_MAX_FIELDS];

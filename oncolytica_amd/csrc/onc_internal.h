@@ -26,13 +26,13 @@ enum onc_phase {
     ONC_PH_OTHER = 9,
 };
 
-// Plan / commit modes of k_plan.
-#define ONC_PLAN_STEP 0
-#define ONC_PLAN_COMMIT 1
 
 // Launchers (all asynchronous on `s`). Grid sizes are chosen by the runtime;
 // `items` (cells per thread of a tile kernel) is 4, 8, 12 or 16 for the sort
 // passes and 4, 8 or 16 for compaction.
+// Plan / commit modes of k_plan.
+#define ONC_PLAN_STEP 0
+#define ONC_PLAN_COMMIT 1
 hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s);
 hipError_t onc_launch_morton(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_hist_only(const onc_kargs& a, int items, int grid, hipStream_t s);
@@ -42,7 +42,6 @@ hipError_t onc_launch_forces(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_stencil(const onc_kargs& a, onc_u32 nblocks, hipStream_t s);
 hipError_t onc_launch_rule_builtin(const onc_rule_args& r, int grid, hipStream_t s);
 hipError_t onc_launch_compact(const onc_kargs& a, int items, int grid, hipStream_t s);
-hipError_t onc_launch_daughters(const onc_kargs& a, int grid, hipStream_t s);
 
 // pure kernels
 hipError_t onc_launch_morton_keys(const float* px, const float* py, const float* pz, onc_u32 n, int X,

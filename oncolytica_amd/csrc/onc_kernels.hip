@@ -144,48 +144,51 @@ __device__ __forceinline__ void lookback_wave64(const onc_u64* status, onc_u32 g
 }
 
 // ---------------------------------------------------------------------------
-// plan / commit (one workgroup): commits the previous step's counters, plans
-// this step's tiles from the live counts, clears the tile counters (the
+// Step bookkeeping, one workgroup for the whole batch, at the start of every
+// step (and, in commit mode, before the host reads a sim). It first closes the
+// step that ran last for every sim that took part in it: the live count
+// becomes survivors + the daughters that fit the capacity (the daughters stay
+// in dau2 after the survivors, see onc_dstate), ids advance, the buffers flip.
+// In step mode it then plans the step: which sims are frozen at their stop
+// step, the prefixes of live cells and of sort / compaction tiles, the
+// per-step scalars of the segment table, and cleared tile counters. (The
 // digit histograms are cleared by the permute kernel, once the sort passes
-// have read them).
+// have read them.)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
     __shared__ onc_u32 s_tmp[8];
-    __shared__ onc_u32 s_carry_t, s_carry_c, s_carry_ct;
+    __shared__ onc_u32 s_carry[3];
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        s_carry_t = 0;
-        s_carry_c = 0;
-        s_carry_ct = 0;
-    }
+    if (tid < 3) s_carry[tid] = 0u;
     __syncthreads();
+    onc_dsim* T = (onc_dsim*)a.sims;  // the plan is the table's only device writer
     for (onc_u32 b0 = 0; b0 < a.nsims; b0 += 256) {
-        onc_u32 s = b0 + tid;
+        const onc_u32 s = b0 + tid;
         onc_u32 nt = 0, nc = 0, nct = 0;
         if (s < a.nsims) {
-            onc_dstate* st = a.sims[s].state;
+            onc_dstate* st = T[s].state;
             if (st->pending) {
+                const onc_u32 ns = st->nsurv, nd = st->ndiv, room = T[s].cap - ns;
+                const onc_u32 avail = nd < room ? nd : room;
+                st->cell_updates += st->n;
+                st->n = ns + avail;
+                st->split = ns;
+                st->next_id += avail;
                 st->parity ^= 1u;
                 st->step += 1u;
-                st->n = st->pend_n;
-                st->next_id = st->pend_next_id;
                 st->pending = 0u;
             }
-            // a sim at its stop step takes part in no kernel of this step
             const bool frozen = st->step >= st->stop;
-            onc_dsim* T = (onc_dsim*)a.sims;  // the plan is the table's only device writer
             T[s].n_live = frozen ? 0u : st->n;
             T[s].par = st->parity;
             T[s].step = st->step;
             T[s].next_id = st->next_id;
+            T[s].split = st->split;
             T[s].frozen = frozen ? 1u : 0u;
             if (mode == ONC_PLAN_STEP && !frozen) {
-                st->pend_n = 0u;
-                st->pend_next_id = st->next_id;
                 st->nsurv = 0u;
                 st->ndiv = 0u;
                 st->pending = 1u;
-                st->cell_updates += st->n;
                 nc = st->n;
                 nt = (nc + a.tile - 1u) / a.tile;
                 nct = (nc + a.ctile - 1u) / a.ctile;
@@ -193,30 +196,30 @@ __global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
         }
         if (mode == ONC_PLAN_STEP) {
             onc_u32 tt, tc, tct;
-            onc_u32 et = block_exscan256(nt, s_tmp, &tt);
-            onc_u32 ec = block_exscan256(nc, s_tmp, &tc);
-            onc_u32 ect = block_exscan256(nct, s_tmp, &tct);
+            const onc_u32 et = block_exscan256(nt, s_tmp, &tt);
+            const onc_u32 ec = block_exscan256(nc, s_tmp, &tc);
+            const onc_u32 ect = block_exscan256(nct, s_tmp, &tct);
             if (s < a.nsims) {
-                a.tile_prefix[s] = s_carry_t + et;
-                a.cell_prefix[s] = s_carry_c + ec;
-                a.ctile_prefix[s] = s_carry_ct + ect;
+                a.tile_prefix[s] = s_carry[0] + et;
+                a.cell_prefix[s] = s_carry[1] + ec;
+                a.ctile_prefix[s] = s_carry[2] + ect;
             }
             __syncthreads();
             if (tid == 0) {
-                s_carry_t += tt;
-                s_carry_c += tc;
-                s_carry_ct += tct;
+                s_carry[0] += tt;
+                s_carry[1] += tc;
+                s_carry[2] += tct;
             }
             __syncthreads();
         }
     }
     if (mode == ONC_PLAN_STEP) {
-        if (tid == 0) {
-            a.tile_prefix[a.nsims] = s_carry_t;
-            a.cell_prefix[a.nsims] = s_carry_c;
-            a.ctile_prefix[a.nsims] = s_carry_ct;
-        }
         for (int i = tid; i < ONC_NCOUNTERS; i += 256) a.counters[i] = 0u;
+        if (tid == 0) {
+            a.tile_prefix[a.nsims] = s_carry[0];
+            a.cell_prefix[a.nsims] = s_carry[1];
+            a.ctile_prefix[a.nsims] = s_carry[2];
+        }
     }
 }
 
@@ -276,19 +279,19 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
             }
             const onc_dsim& S = a.sims[sa];
             const onc_u32 lo = a.cell_prefix[sa];
-            const onc_u32 par = S.par;
-            const float* px = (const float*)S.st[par][0];
-            const float* py = (const float*)S.st[par][1];
-            const float* pz = (const float*)S.st[par][2];
+            const onc_u32 par = S.par, split = S.split;
             float x[ONC_MORTON_CHUNKS], y[ONC_MORTON_CHUNKS], z[ONC_MORTON_CHUNKS];
 #pragma unroll
             for (int q = 0; q < ONC_MORTON_CHUNKS; q++) {
                 const onc_u32 g = base + (onc_u32)q * 256u + (onc_u32)tid;
                 x[q] = y[q] = z[q] = 0.0f;
                 if (g <= last) {
-                    x[q] = px[g - lo];
-                    y[q] = py[g - lo];
-                    z[q] = pz[g - lo];
+                    const onc_u32 i = g - lo;
+                    const bool d = i >= split;
+                    const onc_u32 j = d ? i - split : i;
+                    x[q] = ((const float*)(d ? S.dau2[0] : S.st[par][0]))[j];
+                    y[q] = ((const float*)(d ? S.dau2[1] : S.st[par][1]))[j];
+                    z[q] = ((const float*)(d ? S.dau2[2] : S.st[par][2]))[j];
                 }
             }
 #pragma unroll
@@ -317,9 +320,11 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
                     if (g >= a.cell_prefix[s0 + 1]) s = onc_find_seg(a.cell_prefix, a.nsims, g);
                     const onc_dsim& S = a.sims[s];
                     const onc_u32 i = g - a.cell_prefix[s];
-                    const onc_u32 par = S.par;
-                    const onc_u32 key = onc_cell_key(S, ((const float*)S.st[par][0])[i],
-                                                     ((const float*)S.st[par][1])[i], ((const float*)S.st[par][2])[i]);
+                    const bool d = i >= S.split;
+                    const onc_u32 j = d ? i - S.split : i;
+                    onc_u32* const* src = d ? S.dau2 : S.st[S.par];
+                    const onc_u32 key = onc_cell_key(S, ((const float*)src[0])[j], ((const float*)src[1])[j],
+                                                     ((const float*)src[2])[j]);
                     S.keys[0][i] = key;
                     S.vals[0][i] = i;
                     count(s, cur, key);
@@ -520,7 +525,11 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
         const onc_u32 n = S.n_live;
         const onc_u32 par = S.par;
         const onc_u32* __restrict__ keys = S.keys[P & 1u];
-        const onc_u32 src = S.vals[P & 1u][i];
+        const onc_u32 src0 = S.vals[P & 1u][i];
+        // committed cells: survivors in st[par], then daughters in dau2
+        const bool fromd = src0 >= S.split;
+        const onc_u32 src = fromd ? src0 - S.split : src0;
+        onc_u32* const* from = fromd ? S.dau2 : S.st[par];
         const onc_u32 k = keys[i];
         const onc_u32 kprev = i > 0u ? keys[i - 1u] : ~k;
         const onc_u32 knext = i + 1u < n ? keys[i + 1u] : ~k;
@@ -529,7 +538,7 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
         for (int c0 = 0; c0 < S.n_cols; c0 += 8) {
             onc_u32 v[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) v[q] = (c0 + q < S.n_cols) ? S.st[par][c0 + q][src] : 0u;
+            for (int q = 0; q < 8; q++) v[q] = (c0 + q < S.n_cols) ? from[c0 + q][src] : 0u;
             if (c0 == 0) {
                 px = __uint_as_float(v[0]);
                 py = __uint_as_float(v[1]);
@@ -1075,39 +1084,15 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
             anyd |= sd[k] != NONE;
         }
         compact_copy<ITEMS>(S.st[par], S.st[par ^ 1u], S.n_cols, base, sa);
-        if (anyd) compact_copy<ITEMS>(S.dau, S.dau2, S.n_cols, base, sd);
+        if (anyd) {
+            compact_copy<ITEMS>(S.dau, S.dau2, S.n_cols, base, sd);
+            // daughter ids: next_id + rank among the sim's daughters
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++)
+                if (sd[k] != NONE) S.dau2[ONC_COL_ID_DEV][sd[k]] = S.next_id + sd[k];
+        }
         __syncthreads();
     }
-}
-
-// Append daughters after the survivors (capacity-clamped), assign ids, and
-// record the counters the next plan commits.
-// One thread per potential daughter: daughter k of a sim exists for k < ndiv
-// and ndiv <= n, so the walk over the step's live cells covers them all with
-// the whole chip (most threads exit at once).
-__global__ __launch_bounds__(256) void k_daughters(onc_kargs a) {
-    onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 k) {
-        const onc_dsim& S = a.sims[s];
-        const onc_u32 nsurv = S.state->nsurv, ndiv = S.state->ndiv;
-        const onc_u32 room = S.cap - nsurv;
-        const onc_u32 avail = ndiv < room ? ndiv : room;
-        const onc_u32 nid = S.next_id;
-        if (k == 0u) {
-            S.state->pend_n = nsurv + avail;
-            S.state->pend_next_id = nid + avail;
-        }
-        if (k >= avail) return;
-        const onc_u32 par = S.par;
-        for (int c0 = 0; c0 < S.n_cols; c0 += 8) {
-            onc_u32 v[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) v[q] = (c0 + q < S.n_cols) ? S.dau2[c0 + q][k] : 0u;
-            if (c0 == 0) v[3] = nid + k;
-#pragma unroll
-            for (int q = 0; q < 8; q++)
-                if (c0 + q < S.n_cols) S.st[par ^ 1u][c0 + q][nsurv + k] = v[q];
-        }
-    });
 }
 
 // ---------------------------------------------------------------------------
@@ -1279,10 +1264,6 @@ hipError_t onc_launch_rule_builtin(const onc_rule_args& r, int grid, hipStream_t
 }
 hipError_t onc_launch_compact(const onc_kargs& a, int items, int grid, hipStream_t s) {
     ONC_DISPATCH_ITEMS(items, k_compact, grid, s, a);
-    return hipGetLastError();
-}
-hipError_t onc_launch_daughters(const onc_kargs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_daughters, dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -107,6 +107,8 @@ struct onc_batch {
     hipGraphExec_t exec = nullptr;
     // profiling
     unsigned profile = 0;  // bit k: time phase k (onc_phase)
+    bool prof_on = false;  // events are being recorded for the step being enqueued
+    std::vector<unsigned> ev_mask;  // phases with both events recorded, per profiled step
     std::vector<hipEvent_t> ev;
     int ev_used = 0;
     double phase_us[ONC_PHASES] = {0};
@@ -493,6 +495,7 @@ extern "C" int onc_sim_set_params(onc_sim* s, const float* params, int n) {
     return ONC_OK;
 }
 
+// Device state of a sim with its last step committed.
 static int sim_read_state(onc_sim* s, onc_dstate* st) {
     CKR(batch_commit(s->owner));
     CK(hipMemcpy(st, s->dstate, sizeof(*st), hipMemcpyDeviceToHost));
@@ -510,6 +513,7 @@ extern "C" int onc_sim_upload_cells(onc_sim* s, const void* const* cols, uint32_
         CK(hipMemcpy(s->dsim.st[st.parity][c], cols[c], (size_t)n * 4, hipMemcpyHostToDevice));
     }
     st.n = n;
+    st.split = n;
     st.next_id = next_id;
     CK(hipMemcpy(s->dstate, &st, sizeof(st), hipMemcpyHostToDevice));
     return ONC_OK;
@@ -521,10 +525,14 @@ extern "C" int onc_sim_download_cells(onc_sim* s, void* const* cols, uint32_t ca
     onc_dstate st;
     CKR(sim_read_state(s, &st));
     if (n) *n = st.n;
-    uint32_t m = std::min(cap, st.n);
+    // survivors of the last step, then its daughters (see onc_dstate)
+    const uint32_t m = std::min(cap, st.n), m1 = std::min(m, st.split), m2 = m - m1;
     if (cols && m > 0)
-        for (int c = 0; c < s->desc.n_cols; c++)
-            if (cols[c]) CK(hipMemcpy(cols[c], s->dsim.st[st.parity][c], (size_t)m * 4, hipMemcpyDeviceToHost));
+        for (int c = 0; c < s->desc.n_cols; c++) {
+            if (!cols[c]) continue;
+            if (m1) CK(hipMemcpy(cols[c], s->dsim.st[st.parity][c], (size_t)m1 * 4, hipMemcpyDeviceToHost));
+            if (m2) CK(hipMemcpy((char*)cols[c] + (size_t)m1 * 4, s->dsim.dau2[c], (size_t)m2 * 4, hipMemcpyDeviceToHost));
+        }
     return ONC_OK;
 }
 
@@ -728,6 +736,8 @@ static int batch_rebuild(onc_batch* b) {
     return ONC_OK;
 }
 
+// Drains the batch's stream and commits its sims' last step (the plan
+// kernel in commit mode), so host reads see complete state.
 static int batch_commit(onc_batch* b) {
     if (!b) return ONC_OK;
     CK(hipSetDevice(b->ctx->device));
@@ -770,17 +780,16 @@ static int launch_rule(onc_batch* b) {
     return ONC_OK;
 }
 
-// Profiling: ONC_EV_STEP events per step from a pool, recorded between the
-// phases of eager (non-graph) steps; durations are accumulated at the next
-// sync, so the host never waits inside a run of steps. Slot k is recorded
-// before phase k and slot k + 1 after it, only for the phases being timed (an
+// Profiling: a begin and an end event per timed phase of every eager step,
+// from a pool; durations are accumulated at the next sync, so the host never
+// waits inside a run of steps. Only the phases in the mask get events (an
 // event costs a few microseconds of GPU time, so timing one phase leaves the
 // others nearly undisturbed).
-#define ONC_EV_STEP 10
-static void prof_mark(onc_batch* b, int slot) {
-    const unsigned m = b->profile;
-    const bool want = (slot < ONC_EV_STEP - 1 && ((m >> slot) & 1u)) || (slot > 0 && ((m >> (slot - 1)) & 1u));
-    if (want) hipEventRecord(b->ev[b->ev_used * ONC_EV_STEP + slot], b->stream);
+#define ONC_EV_STEP (2 * ONC_PHASES)
+static void prof_mark(onc_batch* b, int phase, int end) {
+    if (!b->prof_on || !((b->profile >> phase) & 1u)) return;
+    hipEventRecord(b->ev[(size_t)b->ev_used * ONC_EV_STEP + 2 * phase + end], b->stream);
+    if (end) b->ev_mask.back() |= 1u << phase;
 }
 
 static int prof_reserve(onc_batch* b, int nsteps) {
@@ -796,44 +805,50 @@ static int prof_reserve(onc_batch* b, int nsteps) {
 static int prof_collect(onc_batch* b) {
     for (int k = 0; k < b->ev_used; k++) {
         hipEvent_t* e = &b->ev[(size_t)k * ONC_EV_STEP];
-        for (int ph = 0; ph < ONC_EV_STEP - 1; ph++) {
-            if (!((b->profile >> ph) & 1u)) continue;
+        for (int ph = 0; ph < ONC_PHASES; ph++) {
+            if (!((b->ev_mask[k] >> ph) & 1u)) continue;
             float ms = 0.0f;
-            CK(hipEventElapsedTime(&ms, e[ph], e[ph + 1]));
+            CK(hipEventElapsedTime(&ms, e[2 * ph], e[2 * ph + 1]));
             b->phase_us[ph] += 1000.0 * ms;
             b->phase_n[ph] += (ph == ONC_PH_SORT) ? (uint64_t)b->passes : 1u;
         }
     }
     b->ev_used = 0;
+    b->ev_mask.clear();
     return ONC_OK;
 }
 
-
-// Enqueue one timestep for every sim of the batch. Uptake is applied by the
-// permute kernel, so neither the forces nor the sweep depends on the other.
-static int enqueue_step(onc_batch* b, bool profiling) {
+// Enqueue one timestep for every sim of the batch. The plan commits the
+// previous step; uptake is applied by the permute kernel; daughters stay in
+// their compacted buffer until the next step's permute gathers them.
+static int enqueue_step(onc_batch* b) {
     const onc_kargs& a = b->ka;
     hipStream_t st = b->stream;
     const int gt = grid_tiles(b), gc = grid_cells(b);
-    if (profiling) prof_mark(b, 0);
+    prof_mark(b, ONC_PH_PLAN, 0);
     CK(onc_launch_plan(a, ONC_PLAN_STEP, st));
-    if (profiling) prof_mark(b, 1);
+    prof_mark(b, ONC_PH_PLAN, 1);
+    prof_mark(b, ONC_PH_MORTON, 0);
     CK(onc_launch_morton(a, grid_morton(b), st));
-    if (profiling) prof_mark(b, 2);
+    prof_mark(b, ONC_PH_MORTON, 1);
+    prof_mark(b, ONC_PH_SORT, 0);
     for (int p = 0; p < b->passes; p++) CK(onc_launch_sort_pass(a, b->items, p, gt, st));
-    if (profiling) prof_mark(b, 3);
+    prof_mark(b, ONC_PH_SORT, 1);
+    prof_mark(b, ONC_PH_PERMUTE_BINS, 0);
     CK(onc_launch_permute_bins(a, gc, st));
-    if (profiling) prof_mark(b, 4);
+    prof_mark(b, ONC_PH_PERMUTE_BINS, 1);
+    prof_mark(b, ONC_PH_FORCES, 0);
     CK(onc_launch_forces(a, gc, st));
-    if (profiling) prof_mark(b, 5);
+    prof_mark(b, ONC_PH_FORCES, 1);
+    prof_mark(b, ONC_PH_STENCIL, 0);
     CK(onc_launch_stencil(a, b->stencil_blocks, st));
-    if (profiling) prof_mark(b, 6);
+    prof_mark(b, ONC_PH_STENCIL, 1);
+    prof_mark(b, ONC_PH_RULE, 0);
     CKR(launch_rule(b));
-    if (profiling) prof_mark(b, 7);
+    prof_mark(b, ONC_PH_RULE, 1);
+    prof_mark(b, ONC_PH_COMPACT, 0);
     CK(onc_launch_compact(a, b->citems, grid_ctiles(b), st));
-    if (profiling) prof_mark(b, 8);
-    CK(onc_launch_daughters(a, gc, st));
-    if (profiling) prof_mark(b, 9);
+    prof_mark(b, ONC_PH_COMPACT, 1);
     return ONC_OK;
 }
 
@@ -861,21 +876,22 @@ static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
     if (b->dirty) CKR(batch_rebuild(b));
     if (b->sims.empty() || nsteps == 0) return ONC_OK;
     CKR(check_rules(b));
-    if (b->profile && !use_graph) {
-        CKR(prof_reserve(b, nsteps));
-        for (int k = 0; k < nsteps; k++) {
-            CKR(enqueue_step(b, true));
-            b->ev_used++;
-        }
-        return ONC_OK;
-    }
+    const bool profiling = b->profile != 0u && !use_graph;
+    if (profiling) CKR(prof_reserve(b, nsteps));
     if (!use_graph) {
-        for (int k = 0; k < nsteps; k++) CKR(enqueue_step(b, false));
+        for (int k = 0; k < nsteps; k++) {
+            b->prof_on = profiling;
+            if (profiling) b->ev_mask.push_back(0u);
+            const int rc = enqueue_step(b);
+            if (profiling) b->ev_used++;
+            b->prof_on = false;
+            CKR(rc);
+        }
         return ONC_OK;
     }
     if (!b->exec) {
         CK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(b, false);
+        int rc = enqueue_step(b);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(b->stream, &g);
         if (rc != ONC_OK) {
@@ -1004,7 +1020,8 @@ extern "C" int onc_batch_profile(onc_batch* b, int phase_mask) {
     CK(hipSetDevice(b->ctx->device));
     CK(hipStreamSynchronize(b->stream));
     b->ev_used = 0;
-    b->profile = (unsigned)enable & ((1u << (ONC_EV_STEP - 1)) - 1u);
+    b->ev_mask.clear();
+    b->profile = (unsigned)enable & ((1u << ONC_PHASES) - 1u);
     for (int i = 0; i < ONC_PHASES; i++) {
         b->phase_us[i] = 0.0;
         b->phase_n[i] = 0;
