@@ -194,7 +194,7 @@ int onc_batch_set_tuning(onc_batch* b, int sort_items, int compact_items, int st
                          int stencil_by, int stencil_bz);
 /* per-phase HIP-event timing of eager steps on the batch stream: bit k of
    phase_mask times phase k (0 plan, 1 Morton, 2 sort, 3 permute + bins,
-   4 forces, 5 stencil, 6 rule, 7 compaction, 8 daughters); 0 disables */
+   4 forces, 5 stencil, 6 rule, 7 compaction; 8 and 9 are unused); 0 disables */
 int onc_batch_profile(onc_batch* b, int phase_mask);
 #define ONC_PHASES 10
 int onc_batch_phase_times(onc_batch* b, double* us_total, uint64_t* launches, int n);
