@@ -19,6 +19,28 @@ typedef unsigned long long onc_u64;
 #define ONC_DEV_MAX_PASSES 4
 #define ONC_HIST_COPIES 8  // copies of the per-sim digit histograms (atomic spread)
 
+// Device buffers reached through the segment table and the kernel arguments
+// live in global memory. Typed so in device code, their loads and stores
+// compile to global instructions; as generic (flat) instructions they would
+// also count against the wait counter of LDS and scalar loads, so every LDS
+// read or scalar load would first wait for all memory accesses in flight.
+// The host sees plain pointers of the same size and layout (as does the device
+// pass of a translation unit with host code only, which defines ONC_HOST_ONLY).
+//
+// The segment table itself is read through the constant address space: no
+// kernel but the plan writes it (through a cast of its own), so uniform reads
+// of a sim's constants and buffer pointers become scalar loads instead of
+// vector loads whose completion every later memory wait would also count.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(ONC_HOST_ONLY)
+#define ONC_GLOBAL __attribute__((address_space(1)))
+#define ONC_CONST __attribute__((address_space(4)))
+#else
+#define ONC_GLOBAL
+#define ONC_CONST
+#endif
+
+typedef onc_u32 ONC_GLOBAL* onc_gcol;  // a state column in device memory
+
 #define ONC_COL_ID_DEV 3  // the id column of every cell schema
 #define ONC_FLAG_ALIVE 1u
 #define ONC_FLAG_DIV 2u
@@ -70,42 +92,42 @@ struct onc_dsim {
     int n_fields, n_cols;
     onc_dfield fields[ONC_DEV_MAX_FIELDS];
     float params[ONC_DEV_MAX_PARAMS];
-    onc_dstate* state;
-    onc_u32* st[2][ONC_DEV_MAX_COLS];  // cell state columns, two buffers
-    onc_u32* dau[ONC_DEV_MAX_COLS];    // rule output: daughter of cell i at i
-    onc_u32* dau2[ONC_DEV_MAX_COLS];   // daughters compacted in parent order
-    float* fld[2][ONC_DEV_MAX_FIELDS]; // fields, two buffers, x + X (y + Y z)
-    onc_u32* keys[2];
-    onc_u32* vals[2];
-    onc_u32* flags;
+    onc_dstate ONC_GLOBAL* state;
+    onc_u32 ONC_GLOBAL* st[2][ONC_DEV_MAX_COLS];  // cell state columns, two buffers
+    onc_u32 ONC_GLOBAL* dau[ONC_DEV_MAX_COLS];    // rule output: daughter of cell i at i
+    onc_u32 ONC_GLOBAL* dau2[ONC_DEV_MAX_COLS];   // daughters compacted in parent order
+    float ONC_GLOBAL* fld[2][ONC_DEV_MAX_FIELDS]; // fields, two buffers, x + X (y + Y z)
+    onc_u32 ONC_GLOBAL* keys[2];
+    onc_u32 ONC_GLOBAL* vals[2];
+    onc_u32 ONC_GLOBAL* flags;
     // Bin table indexed by Morton key, 16 B per entry {start, end, epoch, 0}.
     // An entry is valid in the step whose epoch (completed steps + 1) it
     // carries; only occupied bins are written, so nothing is cleared per step.
-    onc_u32* bins;
-    float* pos4;  // sorted positions as {x, y, z, 0}, written with the permute
+    onc_u32 ONC_GLOBAL* bins;
+    float ONC_GLOBAL* pos4;  // sorted positions as {x, y, z, 0}, written with the permute
 };
 
 // Arguments of every batched kernel.
 struct onc_kargs {
-    const onc_dsim* sims;
+    const onc_dsim ONC_CONST* sims;
     onc_u32 nsims;
     onc_u32 passes;       // radix passes this step (max over sims)
     onc_u32 tile;         // cells per sort tile
     onc_u32 max_tiles;    // sort tiles the status array holds
-    onc_u32* tile_prefix; // [nsims + 1] sort tile prefix of live counts
+    onc_u32 ONC_GLOBAL* tile_prefix; // [nsims + 1] sort tile prefix of live counts
     onc_u32 ctile;        // cells per compaction tile
     onc_u32 max_ctiles;   // compaction tiles the status array holds
-    onc_u32* ctile_prefix;// [nsims + 1] compaction tile prefix of live counts
-    onc_u32* cell_prefix; // [nsims + 1] prefix of live counts
-    onc_u32* hist;        // [ONC_HIST_COPIES][nsims][4][256] digit histograms
-    onc_u32* counters;    // [16] dynamic tile counters
-    onc_u32* sort_status; // [4][max_tiles][256] look-back words
-    onc_u64* comp_status; // [max_ctiles] look-back words of the compaction scan
-    onc_u32* jobs_prefix; // [njobs + 1] stencil blocks per (sim, field)
-    onc_u32* jobs;        // [njobs] sim << 8 | field
+    onc_u32 ONC_GLOBAL* ctile_prefix;// [nsims + 1] compaction tile prefix of live counts
+    onc_u32 ONC_GLOBAL* cell_prefix; // [nsims + 1] prefix of live counts
+    onc_u32 ONC_GLOBAL* hist;        // [ONC_HIST_COPIES][nsims][4][256] digit histograms
+    onc_u32 ONC_GLOBAL* counters;    // [16] dynamic tile counters
+    onc_u32 ONC_GLOBAL* sort_status; // [4][max_tiles][256] look-back words
+    onc_u64 ONC_GLOBAL* comp_status; // [max_ctiles] look-back words of the compaction scan
+    onc_u32 ONC_GLOBAL* jobs_prefix; // [njobs + 1] stencil blocks per (sim, field)
+    onc_u32 ONC_GLOBAL* jobs;        // [njobs] sim << 8 | field
     onc_u32 njobs;
     int sbx, sby, sbz;    // stencil block shape
-    onc_u32* err;         // sticky error word (bounded waits that expired)
+    onc_u32 ONC_GLOBAL* err;         // sticky error word (bounded waits that expired)
 };
 
 // ---- small helpers ---------------------------------------------------------
@@ -389,8 +411,8 @@ __device__ __forceinline__ void onc_for_neighbor_pos4(const onc_dsim& S, onc_u32
 // Arguments of a rule kernel (built-in or hiprtc module): one thread per live
 // cell of every sim in the segment table.
 struct onc_rule_args {
-    const onc_dsim* sims;
+    const onc_dsim ONC_CONST* sims;
     onc_u32 nsims;
     onc_u32 passes;
-    const onc_u32* cell_prefix;
+    const onc_u32 ONC_GLOBAL* cell_prefix;
 };

@@ -155,13 +155,16 @@ __device__ __forceinline__ void lookback_wave64(const onc_u64* status, onc_u32 g
 // digit histograms are cleared by the permute kernel, once the sort passes
 // have read them.)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode) {
+__global__ __launch_bounds__(256) void k_plan(onc_kargs a, int mode, onc_dsim* table) {
     __shared__ onc_u32 s_tmp[8];
     __shared__ onc_u32 s_carry[3];
     const int tid = threadIdx.x;
     if (tid < 3) s_carry[tid] = 0u;
     __syncthreads();
-    onc_dsim* T = (onc_dsim*)a.sims;  // the plan is the table's only device writer
+    // The plan is the table's only device writer. It writes through `table`
+    // (the same memory as a.sims): the other kernels read a.sims as constant
+    // memory, and stores derived from a constant-memory pointer are undefined.
+    onc_dsim* T = table;
     for (onc_u32 b0 = 0; b0 < a.nsims; b0 += 256) {
         const onc_u32 s = b0 + tid;
         onc_u32 nt = 0, nc = 0, nct = 0;
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
                     const onc_u32 i = g - a.cell_prefix[s];
                     const bool d = i >= S.split;
                     const onc_u32 j = d ? i - S.split : i;
-                    onc_u32* const* src = d ? S.dau2 : S.st[S.par];
+                    onc_gcol const* src = d ? S.dau2 : S.st[S.par];
                     const onc_u32 key = onc_cell_key(S, ((const float*)src[0])[j], ((const float*)src[1])[j],
                                                      ((const float*)src[2])[j]);
                     S.keys[0][i] = key;
@@ -407,7 +410,7 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
         const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.tile_prefix[s];
-        const onc_u32 n = S.state->n;  // (measured faster here than the table copy)
+        const onc_u32 n = S.n_live;
         const onc_u32 base = t * (onc_u32)TILE;
         const onc_u32 cnt = min((onc_u32)TILE, n - base);
         const onc_u32* kin = S.keys[pass & 1];
@@ -529,7 +532,7 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
         // committed cells: survivors in st[par], then daughters in dau2
         const bool fromd = src0 >= S.split;
         const onc_u32 src = fromd ? src0 - S.split : src0;
-        onc_u32* const* from = fromd ? S.dau2 : S.st[par];
+        onc_gcol const* from = fromd ? S.dau2 : S.st[par];
         const onc_u32 k = keys[i];
         const onc_u32 kprev = i > 0u ? keys[i - 1u] : ~k;
         const onc_u32 knext = i + 1u < n ? keys[i + 1u] : ~k;
@@ -586,11 +589,12 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // nine bin entries are loaded together, the non-empty ranges are compacted
 // into the thread's LDS slots, and a cursor walks them eight candidates at a
 // time (the eight position loads are in flight together); the cursor only
-// reads LDS when it moves to the next range.
-template <class F>
+// reads LDS when it moves to the next range. `batch_end()` runs after every
+// batch of eight.
+template <class F, class G>
 __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
                                             onc_u32 self, const float4* __restrict__ pos4, onc_u32 (*s_rs)[256],
-                                            onc_u32 (*s_re)[256], F pair) {
+                                            onc_u32 (*s_re)[256], F pair, G batch_end) {
     const int tid = threadIdx.x;
     const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
               vz = onc_vox1(pz, S.dx, S.dims[2]);
@@ -642,6 +646,7 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, fl
 #pragma unroll
             for (int u = 0; u < 8; u++)
                 if (jj[u] != 0xFFFFFFFFu && jj[u] != self) pair(p[u], jj[u]);
+            batch_end();
         }
     }
 }
@@ -699,18 +704,25 @@ __global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
         };
         // Phase A: cheap distance filter over the 27 bins; only pairs inside
         // the interaction radius reach the expensive (sqrt, divide) path, so
-        // lanes diverge on short lists instead of on every candidate.
-        forces_walk(S, epoch, px, py, pz, i, p4, s_rs, s_re, [&](const float4& o, onc_u32 j) {
-            const float ddx = px - o.x;
-            const float ddy = py - o.y;
-            const float ddz = pz - o.z;
-            float d2 = ddx * ddx + ddy * ddy;
-            d2 = d2 + ddz * ddz;
-            if (!(d2 < R2) || d2 == 0.0f) return;
-            if (npair == ONC_PAIR_LIST) flush();
-            s_pairs[npair][tid] = j;
-            npair++;
-        });
+        // lanes diverge on short lists instead of on every candidate. A list
+        // holds at most 8 entries between batches (a batch adds at most 8);
+        // once any lane of the wave holds more, every lane still walking
+        // flushes together. A flush taken by one lane at a time would run
+        // the wave through the pair law once per lane.
+        forces_walk(S, epoch, px, py, pz, i, p4, s_rs, s_re,
+                    [&](const float4& o, onc_u32 j) {
+                        const float ddx = px - o.x;
+                        const float ddy = py - o.y;
+                        const float ddz = pz - o.z;
+                        float d2 = ddx * ddx + ddy * ddy;
+                        d2 = d2 + ddz * ddz;
+                        if (!(d2 < R2) || d2 == 0.0f) return;
+                        s_pairs[npair][tid] = j;
+                        npair++;
+                    },
+                    [&]() {
+                        if (__ballot(npair > ONC_PAIR_LIST / 2) != 0ull) flush();
+                    });
         flush();
         const float mob = S.mobility, md = S.max_disp;
         float dpx = mob * Fx, dpy = mob * Fy, dpz = mob * Fz;
@@ -919,8 +931,8 @@ __global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
     onc_for_cells(r.cell_prefix, r.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = r.sims[s];
         const onc_u32 par = S.par;
-        onc_u32* const* T = S.st[par ^ 1u];
-        onc_u32* const* C = S.st[par];
+        onc_gcol const* T = S.st[par ^ 1u];
+        onc_gcol const* C = S.st[par];
         float px = ((const float*)C[0])[i], py = ((const float*)C[1])[i], pz = ((const float*)C[2])[i];
         const onc_u32 id = T[3][i];
         float age = __uint_as_float(T[4][i]);
@@ -978,7 +990,7 @@ __global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
 // dst[c][slot[k]] = src[c][base + k * 256 + tid] for every column c and
 // every item k whose slot is not NONE.
 template <int ITEMS>
-__device__ __forceinline__ void compact_copy(onc_u32* const* src, onc_u32* const* dst, int n_cols, onc_u32 base,
+__device__ __forceinline__ void compact_copy(onc_gcol const* src, onc_gcol const* dst, int n_cols, onc_u32 base,
                                              const onc_u32* slot) {
     const int tid = threadIdx.x;
     for (int c0 = 0; c0 < n_cols; c0 += 4) {
@@ -986,14 +998,14 @@ __device__ __forceinline__ void compact_copy(onc_u32* const* src, onc_u32* const
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (c0 + q >= n_cols) break;
-            const onc_u32* __restrict__ sp = src[c0 + q] + base;
+            const onc_u32 ONC_GLOBAL* __restrict__ sp = src[c0 + q] + base;
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) v[q][k] = slot[k] != 0xFFFFFFFFu ? sp[k * 256 + tid] : 0u;
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             if (c0 + q >= n_cols) break;
-            onc_u32* __restrict__ dp = dst[c0 + q];
+            onc_gcol __restrict__ dp = dst[c0 + q];
 #pragma unroll
             for (int k = 0; k < ITEMS; k++)
                 if (slot[k] != 0xFFFFFFFFu) dp[slot[k]] = v[q][k];
@@ -1217,7 +1229,7 @@ __global__ __launch_bounds__(256) void k_exscan(const onc_u32* __restrict__ in, 
 // launchers
 // ---------------------------------------------------------------------------
 hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s) {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, s, a, mode);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, s, a, mode, (onc_dsim*)a.sims);
     return hipGetLastError();
 }
 

@@ -7,6 +7,11 @@
 // (framework/src/oncolytica/gpu/runtime/_gpu_backend.py:113-171, 407-432):
 // one HIP stream per batch, one launch per phase for all of its sims, live
 // counts kept on the device.
+// Host code only: the buffer pointers of the segment table stay plain
+// pointers in this file's device pass too (it stores host-side values into
+// them); the kernels' translation unit sees them as global-memory pointers.
+#define ONC_HOST_ONLY 1
+
 #include <hip/hiprtc.h>
 
 #include <algorithm>

@@ -1178,9 +1178,9 @@ class Translator:
                     stl.append(f"    c[{k}][i] = v.{col.name};")
         out.append(f"__device__ __forceinline__ {cs.cname} {cs.cname}_defaults() {{\n    {cs.cname} r;\n"
                    + "\n".join(self._defaults_expr(cs)) + "\n    r._id = 0u;\n    r._idx = 0xFFFFFFFFu;\n    return r;\n}\n")
-        out.append(f"__device__ __forceinline__ {cs.cname} {cs.cname}_load(const onc_dsim& S, onc_u32* const* c, onc_u32 i) {{\n"
+        out.append(f"__device__ __forceinline__ {cs.cname} {cs.cname}_load(const onc_dsim& S, onc_gcol const* c, onc_u32 i) {{\n"
                    f"    {cs.cname} r;\n" + "\n".join(ld) + "\n    r._idx = i;\n    return r;\n}\n")
-        out.append(f"__device__ __forceinline__ void {cs.cname}_store(const onc_dsim& S, onc_u32* const* c, onc_u32 i, {cs.cname} v) {{\n"
+        out.append(f"__device__ __forceinline__ void {cs.cname}_store(const onc_dsim& S, onc_gcol const* c, onc_u32 i, {cs.cname} v) {{\n"
                    "    v.pos.x = onc_clampf(v.pos.x, 0.0f, (float)S.dims[0] * S.dx);\n"
                    "    v.pos.y = onc_clampf(v.pos.y, 0.0f, (float)S.dims[1] * S.dx);\n"
                    "    v.pos.z = onc_clampf(v.pos.z, 0.0f, (float)S.dims[2] * S.dx);\n"
