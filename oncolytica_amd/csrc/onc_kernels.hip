@@ -582,6 +582,7 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // sweep.
 // ---------------------------------------------------------------------------
 #define ONC_PAIR_LIST 16
+#define ONC_FORCES_BLOCK 64  // one wave per block
 
 // Candidates of the 27 Moore bins around (px, py, pz), z-plane by z-plane, in
 // template.wgsl:11-21 order (x fastest) and bin order inside a bin, skipping
@@ -593,8 +594,8 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // batch of eight.
 template <class F, class G>
 __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
-                                            onc_u32 self, const float4* __restrict__ pos4, onc_u32 (*s_rs)[256],
-                                            onc_u32 (*s_re)[256], F pair, G batch_end) {
+                                            onc_u32 self, const float4* __restrict__ pos4, onc_u32 (*s_rs)[ONC_FORCES_BLOCK],
+                                            onc_u32 (*s_re)[ONC_FORCES_BLOCK], F pair, G batch_end) {
     const int tid = threadIdx.x;
     const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
               vz = onc_vox1(pz, S.dx, S.dims[2]);
@@ -651,11 +652,11 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, fl
     }
 }
 
-__global__ __launch_bounds__(256) void k_forces(onc_kargs a) {
+__global__ __launch_bounds__(ONC_FORCES_BLOCK) void k_forces(onc_kargs a) {
     // per-thread list of interacting neighbours, [slot][thread] (bank-conflict free)
-    __shared__ onc_u32 s_pairs[ONC_PAIR_LIST][256];
+    __shared__ onc_u32 s_pairs[ONC_PAIR_LIST][ONC_FORCES_BLOCK];
     // per-thread non-empty bin ranges of the current z-plane
-    __shared__ onc_u32 s_rs[9][256], s_re[9][256];
+    __shared__ onc_u32 s_rs[9][ONC_FORCES_BLOCK], s_re[9][ONC_FORCES_BLOCK];
     const int tid = threadIdx.x;
     onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = a.sims[s];
@@ -1262,7 +1263,7 @@ hipError_t onc_launch_permute_bins(const onc_kargs& a, int grid, hipStream_t s) 
     return hipGetLastError();
 }
 hipError_t onc_launch_forces(const onc_kargs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_forces, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_forces, dim3(grid), dim3(ONC_FORCES_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 hipError_t onc_launch_stencil(const onc_kargs& a, onc_u32 nblocks, hipStream_t s) {

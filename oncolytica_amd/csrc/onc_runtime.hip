@@ -758,6 +758,12 @@ static int grid_cells(onc_batch* b) {
     uint64_t lim = (uint64_t)b->ctx->cus * 8;
     return (int)std::max<uint64_t>(1, std::min(need, lim));
 }
+// Forces: one-wave blocks, one per 64 cells of capacity (up to 65535), so
+// that each wave walks about one chunk and a slow chunk (dense bins) holds a
+// single wave slot while the dispatcher keeps the others busy.
+static int grid_forces(onc_batch* b) {
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>((b->total_cap + 63) / 64, 65535));
+}
 // Tile kernels: one block per tile the sims' capacity can need (blocks past
 // this step's live tiles leave at once), up to a cap beyond which blocks loop.
 static int grid_tiles(onc_batch* b) {
@@ -843,7 +849,7 @@ static int enqueue_step(onc_batch* b) {
     CK(onc_launch_permute_bins(a, gc, st));
     prof_mark(b, ONC_PH_PERMUTE_BINS, 1);
     prof_mark(b, ONC_PH_FORCES, 0);
-    CK(onc_launch_forces(a, gc, st));
+    CK(onc_launch_forces(a, grid_forces(b), st));
     prof_mark(b, ONC_PH_FORCES, 1);
     prof_mark(b, ONC_PH_STENCIL, 0);
     CK(onc_launch_stencil(a, b->stencil_blocks, st));
