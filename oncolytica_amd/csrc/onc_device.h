@@ -285,6 +285,18 @@ __device__ __forceinline__ onc_u32 onc_xcd_block(onc_u32 bid, onc_u32 nblocks) {
     return xcd * q + (xcd < r ? xcd : r) + idx;
 }
 
+// Interleaved block order for work of uneven cost: XCD k takes groups of G
+// consecutive logical block ids, the groups round-robin over the 8 XCDs, so
+// every XCD walks the whole id range at the same pace (a contiguous eighth per
+// XCD would leave the XCDs holding the dense sims of a batch working alone at
+// the end). Blocks past the last full round of 8 G keep their own id.
+__device__ __forceinline__ onc_u32 onc_xcd_group(onc_u32 bid, onc_u32 nblocks, onc_u32 G) {
+    const onc_u32 full = nblocks / (8u * G) * (8u * G);
+    if (bid >= full) return bid;
+    const onc_u32 xcd = bid & 7u, j = bid >> 3;
+    return ((j / G) * 8u + xcd) * G + j % G;
+}
+
 // Persistent walk over the live cells of every sim of the segment table, one
 // wave64 chunk of 64 consecutive cells at a time. Each wave takes a contiguous
 // range of chunks (waves of an XCD take adjacent ranges), so a wave's Morton
@@ -292,12 +304,16 @@ __device__ __forceinline__ onc_u32 onc_xcd_block(onc_u32 bid, onc_u32 nblocks) {
 // case) the sim index is wave-uniform, so the segment-table entry and the
 // column pointers are read with scalar loads. `f(s, i)` gets the sim and the
 // cell's index in it.
-template <class F>
+// With XCD_GROUP > 0 the blocks are spread by onc_xcd_group instead (for
+// chunks of uneven cost).
+template <int XCD_GROUP = 0, class F>
 __device__ __forceinline__ void onc_for_cells(const onc_u32* cell_prefix, onc_u32 nsims, F f) {
     const onc_u32 total = cell_prefix[nsims];
     const onc_u32 lane = threadIdx.x & 63u;
     const onc_u32 wpb = blockDim.x >> 6;
-    const onc_u32 wave = onc_uniform(onc_xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6));
+    const onc_u32 blk = XCD_GROUP > 0 ? onc_xcd_group(blockIdx.x, gridDim.x, (onc_u32)XCD_GROUP)
+                                      : onc_xcd_block(blockIdx.x, gridDim.x);
+    const onc_u32 wave = onc_uniform(blk * wpb + (threadIdx.x >> 6));
     const onc_u32 nwaves = gridDim.x * wpb;
     const onc_u32 nchunks = (total + 63u) >> 6;
     const onc_u32 c0 = (onc_u32)(((unsigned long long)nchunks * wave) / nwaves);

@@ -583,6 +583,7 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // ---------------------------------------------------------------------------
 #define ONC_PAIR_LIST 16
 #define ONC_FORCES_BLOCK 64  // one wave per block
+#define ONC_FORCES_XCD_GROUP 8  // chunks of very uneven cost: interleave the XCDs
 
 // Candidates of the 27 Moore bins around (px, py, pz), z-plane by z-plane, in
 // template.wgsl:11-21 order (x fastest) and bin order inside a bin, skipping
@@ -658,7 +659,7 @@ __global__ __launch_bounds__(ONC_FORCES_BLOCK) void k_forces(onc_kargs a) {
     // per-thread non-empty bin ranges of the current z-plane
     __shared__ onc_u32 s_rs[9][ONC_FORCES_BLOCK], s_re[9][ONC_FORCES_BLOCK];
     const int tid = threadIdx.x;
-    onc_for_cells(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
+    onc_for_cells<ONC_FORCES_XCD_GROUP>(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = a.sims[s];
         const onc_u32 par = S.par;
         const onc_u32 epoch = S.step + 1u;
