@@ -41,8 +41,9 @@ def algorithmic_bytes(phase: str, n_mean: float, nvox: int, n_fields: int, cell_
     """Algorithmic HBM bytes of ONE launch of a phase (units: cells/voxels of
     the timed steps). Per-unit figures: Morton 20 B/cell; radix pass 16 B/cell;
     permute 2S + key 4 B/cell + 8 B per occupied bin; forces 12 B read + 12 B
-    written + 4 B key per cell; stencil 8 B/voxel/field; rule 2S + 4F + 4 B
-    flags per cell; compaction 4 B flag + 8 B scan + 2S per cell."""
+    written + 4 B key per cell; stencil 8 B/voxel/field; forces and stencil
+    in one launch: the sum of the two; rule 2S + 4F + 4 B flags per cell;
+    compaction 4 B flag + 8 B scan + 2S per cell."""
     S = cell_bytes
     return {
         "morton": 20.0 * n_mean,
@@ -50,6 +51,7 @@ def algorithmic_bytes(phase: str, n_mean: float, nvox: int, n_fields: int, cell_
         "permute_bins": (2 * S + 4) * n_mean + 8.0 * occ_bins,
         "forces": 28.0 * n_mean,
         "stencil": 8.0 * nvox * n_fields,
+        "forces_stencil": 28.0 * n_mean + 8.0 * nvox * n_fields,
         "rule": (2 * S + 4 * n_fields + 4) * n_mean,
         "compact": (4 + 8 + 2 * S) * n_mean,
     }.get(phase)
@@ -316,7 +318,7 @@ PROFILE_RUNS = {
 }
 KERNEL_PHASE = {"k_plan": "plan", "k_morton": "morton", "k_sort_pass": "sort", "k_permute_bins": "permute_bins",
                 "k_forces": "forces", "k_stencil": "stencil", "k_rule_tumor": "rule", "k_compact": "compact",
-                "k_daughters": "daughters", "k_copy_u32": "copy"}
+                "k_forces_stencil": "forces_stencil", "k_copy_u32": "copy"}
 COPY_BYTES = 4 * (64 << 20)  # one --calibrate copy launch reads and writes this many bytes
 
 
@@ -443,10 +445,11 @@ def main():
 
     import ctypes as C
 
-    def fresh():
+    def fresh(fused=True):
         sim = Sim(ctx, desc)
         sim.upload_cells(cols0)
         b = Batch(ctx)
+        b.set_fused(fused)
         b.add(sim)
         return sim, b
 
@@ -463,18 +466,25 @@ def main():
     wb.close()
     ws.close()
 
-    # profiled pass: the same K steps with HIP events around every phase
-    # (per-phase table; events cost GPU time, so this pass is not the timed one)
-    psim, pbatch = fresh()
-    pbatch.profile(True)
-    pbatch.sync()
-    pcu0 = cell_updates(psim)
-    pbatch.step(args.steps, graph=False)
-    pbatch.sync()
-    phases = pbatch.phase_times()
-    pn_mean = (cell_updates(psim) - pcu0) / max(1, args.steps)
-    pbatch.close()
-    psim.close()
+    # profiled passes: the same K steps with HIP events around every phase
+    # (per-phase tables; events cost GPU time, so these passes are not the
+    # timed one), with forces and the sweep in one launch (the default) and
+    # in two launches (their separate times; the sweep's own roofline)
+    def profiled(fused):
+        psim, pbatch = fresh(fused)
+        pbatch.profile(True)
+        pbatch.sync()
+        pcu0 = cell_updates(psim)
+        pbatch.step(args.steps, graph=False)
+        pbatch.sync()
+        ph = pbatch.phase_times()
+        nm = (cell_updates(psim) - pcu0) / max(1, args.steps)
+        pbatch.close()
+        psim.close()
+        return ph, nm
+
+    phases, pn_mean = profiled(True)
+    phases_sep, _ = profiled(False)
 
     def occupied_bins(sim, n_mean):
         import numpy as np
@@ -489,8 +499,9 @@ def main():
     for name, (us, nl) in phases.items():
         if nl == 0:
             continue
-        byts = algorithmic_bytes(name, pn_mean, w.nvox, len(w.fields), W.CELL_BYTES, pn_mean)
         table[name] = {"total_ms": us / 1000.0, "launches": nl, "avg_us": us / nl}
+    table_sep = {name: {"total_ms": us / 1000.0, "launches": nl, "avg_us": us / nl}
+                 for name, (us, nl) in phases_sep.items() if nl and name in ("forces", "stencil")}
     dom = max((k for k in table if algorithmic_bytes(k, 1.0, 1, 1, 1, 1.0)), key=lambda k: table[k]["total_ms"])
 
     # timed region: steps 1..K of the workload from its seeded initial state,
@@ -512,6 +523,18 @@ def main():
     n_final, _, steps_done = sim.counts()
     occ = occupied_bins(sim, n_mean)
 
+    # the same timed region with forces and the sweep as two launches
+    ssim, sbatch = fresh(False)
+    sbatch.sync()
+    scu0 = cell_updates(ssim)
+    s0 = time.perf_counter()
+    sbatch.step(args.steps, graph=False)
+    sbatch.sync()
+    s1 = time.perf_counter()
+    sep_value = (cell_updates(ssim) - scu0) / (s1 - s0)
+    sbatch.close()
+    ssim.close()
+
     # the same steps replayed as one captured hipGraph per step (the first
     # step captures and is not timed)
     gsim, gbatch = fresh()
@@ -527,17 +550,20 @@ def main():
     gbatch.close()
     gsim.close()
 
-    for name, row in table.items():
-        byts = algorithmic_bytes(name, n_mean, w.nvox, len(w.fields), W.CELL_BYTES, occ)
-        gbs = (byts / (row["avg_us"] * 1e-6) / 1e9) if byts else None
-        row.update({"alg_bytes": byts, "GBps": gbs})
-        _log(f"{name:13s} {row['total_ms']:9.2f} ms {row['launches']:6d} launches {row['avg_us']:9.2f} us/launch"
-             + (f" {gbs:8.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:5.1f}% of peak)" if gbs else ""))
+    for tag, tb in (("", table), ("separate launches: ", table_sep)):
+        for name, row in tb.items():
+            byts = algorithmic_bytes(name, n_mean, w.nvox, len(w.fields), W.CELL_BYTES, occ)
+            gbs = (byts / (row["avg_us"] * 1e-6) / 1e9) if byts else None
+            row.update({"alg_bytes": byts, "GBps": gbs})
+            _log(f"{tag}{name:14s} {row['total_ms']:9.2f} ms {row['launches']:6d} launches "
+                 f"{row['avg_us']:9.2f} us/launch"
+                 + (f" {gbs:8.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:5.1f}% of peak)" if gbs else ""))
     dom_avg = dom_us / max(1, dom_n)
     dom_bytes = algorithmic_bytes(dom, n_mean, w.nvox, len(w.fields), W.CELL_BYTES, occ)
     dom_gbs = dom_bytes / (dom_avg * 1e-6) / 1e9
     _log(f"timed region: {value / 1e6:.0f} M cell-updates/s, dominant kernel {dom} {dom_avg:.2f} us/launch "
-         f"{dom_gbs:.0f} GB/s; graph replay {(gcu1 - gcu0) / (g1 - g0) / 1e6:.0f} M cell-updates/s")
+         f"{dom_gbs:.0f} GB/s; graph replay {(gcu1 - gcu0) / (g1 - g0) / 1e6:.0f} M cell-updates/s; "
+         f"forces and sweep as two launches {sep_value / 1e6:.0f} M cell-updates/s")
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_large_1m_256.json")
     if os.path.exists(tf):
@@ -575,6 +601,8 @@ def main():
         "graph_value": None,
         "roofline": roof,
         "phases": table,
+        "phases_separate_launches": table_sep,
+        "value_separate_launches": sep_value,
         "cpu_baseline": cpu,
         "ensemble": ens,
         "serving": served,

@@ -192,9 +192,21 @@ int onc_batch_sync(onc_batch* b);
    tile (4, 8, 12 or 16), and the stencil block (x, y with x*y == 256, z-chunk) */
 int onc_batch_set_tuning(onc_batch* b, int sort_items, int compact_items, int stencil_bx,
                          int stencil_by, int stencil_bz);
+/* fused = 1 (the default): the neighbour forces and the 7-point sweep run as
+   one launch, side by side on the same CUs (the force walk is bound by gather
+   latency and ALU, the sweep by HBM); 0: two launches, forces then the sweep.
+   Results are identical either way. (The reference runs them as separate
+   compute passes, gpu/runtime/_gpu_backend.py:127-156.) */
+int onc_batch_set_fused(onc_batch* b, int fused);
+/* cap on the one-wave blocks of the neighbour-force walk: 0 (the default) =
+   one per 64 cells of the batch's capacity; otherwise a positive multiple of
+   8, and each wave strides through its XCD's 64-cell chunks. Results are
+   identical for every value. */
+int onc_batch_set_forces_waves(onc_batch* b, int waves);
 /* per-phase HIP-event timing of eager steps on the batch stream: bit k of
    phase_mask times phase k (0 plan, 1 Morton, 2 sort, 3 permute + bins,
-   4 forces, 5 stencil, 6 rule, 7 compaction; 8 and 9 are unused); 0 disables */
+   4 forces, 5 stencil, 6 rule, 7 compaction, 8 forces + stencil in one launch;
+   9 is unused); 0 disables */
 int onc_batch_profile(onc_batch* b, int phase_mask);
 #define ONC_PHASES 10
 int onc_batch_phase_times(onc_batch* b, double* us_total, uint64_t* launches, int n);

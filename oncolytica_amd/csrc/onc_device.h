@@ -127,6 +127,7 @@ struct onc_kargs {
     onc_u32 ONC_GLOBAL* jobs;        // [njobs] sim << 8 | field
     onc_u32 njobs;
     int sbx, sby, sbz;    // stencil block shape
+    onc_u32 fs_sgroups;   // fused forces + sweep launch: groups of 8 blocks that sweep
     onc_u32 ONC_GLOBAL* err;         // sticky error word (bounded waits that expired)
 };
 
@@ -306,15 +307,12 @@ __device__ __forceinline__ onc_u32 onc_xcd_group(onc_u32 bid, onc_u32 nblocks, o
 // cell's index in it.
 // With XCD_GROUP > 0 the blocks are spread by onc_xcd_group instead (for
 // chunks of uneven cost).
-template <int XCD_GROUP = 0, class F>
-__device__ __forceinline__ void onc_for_cells(const onc_u32* cell_prefix, onc_u32 nsims, F f) {
+// `wave` / `nwaves`: this wave's index among all waves taking part.
+template <class F>
+__device__ __forceinline__ void onc_for_cells_wave(const onc_u32* cell_prefix, onc_u32 nsims, onc_u32 wave,
+                                                   onc_u32 nwaves, F f) {
     const onc_u32 total = cell_prefix[nsims];
     const onc_u32 lane = threadIdx.x & 63u;
-    const onc_u32 wpb = blockDim.x >> 6;
-    const onc_u32 blk = XCD_GROUP > 0 ? onc_xcd_group(blockIdx.x, gridDim.x, (onc_u32)XCD_GROUP)
-                                      : onc_xcd_block(blockIdx.x, gridDim.x);
-    const onc_u32 wave = onc_uniform(blk * wpb + (threadIdx.x >> 6));
-    const onc_u32 nwaves = gridDim.x * wpb;
     const onc_u32 nchunks = (total + 63u) >> 6;
     const onc_u32 c0 = (onc_u32)(((unsigned long long)nchunks * wave) / nwaves);
     const onc_u32 c1 = (onc_u32)(((unsigned long long)nchunks * (wave + 1u)) / nwaves);
@@ -330,6 +328,48 @@ __device__ __forceinline__ void onc_for_cells(const onc_u32* cell_prefix, onc_u3
             f(s, g - cell_prefix[s]);
         }
     }
+}
+
+// Walk for a grid of nblk one-wave blocks (a multiple of 8) that may hold
+// fewer waves than chunks.
+// Chunks go to XCDs in groups of 8 (512 consecutive cells, one spatial
+// neighbourhood): group g belongs to XCD g % 8, so every XCD walks the whole
+// cell range at the same pace. Wave i of XCD x (block id 8 i + x, the
+// dispatcher's round-robin) takes XCD x's chunks i, i + W_x, i + 2 W_x, ...,
+// where the m-th chunk of XCD x is 8 (x + 8 floor(m / 8)) + m % 8: with at
+// least one wave per chunk each wave takes one chunk; with fewer, a wave's
+// chunks are strided through its XCD's share, which spreads dense and sparse
+// regions evenly over the waves.
+template <class F>
+__device__ __forceinline__ void onc_for_chunks_xcd(const onc_u32* cell_prefix, onc_u32 nsims, onc_u32 blk,
+                                                   onc_u32 nblk, F f) {
+    const onc_u32 total = cell_prefix[nsims];
+    const onc_u32 lane = threadIdx.x & 63u;
+    const onc_u32 nchunks = (total + 63u) >> 6;
+    const onc_u32 x = blk & 7u, i = blk >> 3;
+    const onc_u32 wx = nblk >> 3;  // nblk is a multiple of 8 (>= 8): every XCD class has waves
+    for (onc_u32 m = i;; m += wx) {
+        const onc_u32 c = 8u * (x + 8u * (m >> 3)) + (m & 7u);
+        if (c >= nchunks) break;
+        const onc_u32 base = c * 64u;
+        const onc_u32 g = base + lane;
+        const int s0 = (int)onc_uniform((onc_u32)onc_find_seg(cell_prefix, nsims, base));
+        const onc_u32 lo = cell_prefix[s0], hi = cell_prefix[s0 + 1];
+        if (base + 63u < hi) {
+            f(s0, g - lo);
+        } else if (g < total) {
+            const int s = onc_find_seg(cell_prefix, nsims, g);
+            f(s, g - cell_prefix[s]);
+        }
+    }
+}
+
+template <int XCD_GROUP = 0, class F>
+__device__ __forceinline__ void onc_for_cells(const onc_u32* cell_prefix, onc_u32 nsims, F f) {
+    const onc_u32 wpb = blockDim.x >> 6;
+    const onc_u32 blk = XCD_GROUP > 0 ? onc_xcd_group(blockIdx.x, gridDim.x, (onc_u32)XCD_GROUP)
+                                      : onc_xcd_block(blockIdx.x, gridDim.x);
+    onc_for_cells_wave(cell_prefix, nsims, onc_uniform(blk * wpb + (threadIdx.x >> 6)), gridDim.x * wpb, f);
 }
 
 __device__ __forceinline__ onc_u32 onc_field_index(const onc_dsim& S, float px, float py, float pz) {

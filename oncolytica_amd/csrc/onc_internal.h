@@ -12,6 +12,16 @@
 
 #define ONC_BLOCK 256
 
+// Width in float4 columns of a float4 stencil tile: the smallest power of two
+// >= X/4, capped at sbx (a power of two <= 256). Lanes of one row segment of
+// min(width, 64) lanes hold consecutive float4 of one row, so x halos pass
+// between neighbouring lanes; the tile is 256 / width rows tall.
+__host__ __device__ inline int onc_stencil_tw(int X4, int sbx) {
+    int p = 1;
+    while (p < X4 && p < sbx) p <<= 1;
+    return p;
+}
+
 // Phase ids for per-phase timing (onc_batch_phase_times).
 enum onc_phase {
     ONC_PH_PLAN = 0,
@@ -22,7 +32,7 @@ enum onc_phase {
     ONC_PH_STENCIL = 5,
     ONC_PH_RULE = 6,
     ONC_PH_COMPACT = 7,
-    ONC_PH_DAUGHTERS = 8,
+    ONC_PH_FORCES_STENCIL = 8,
     ONC_PH_OTHER = 9,
 };
 
@@ -40,6 +50,10 @@ hipError_t onc_launch_sort_pass(const onc_kargs& a, int items, int pass, int gri
 hipError_t onc_launch_permute_bins(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_forces(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_stencil(const onc_kargs& a, onc_u32 nblocks, hipStream_t s);
+// forces + sweep in one launch of 8 * ngroups one-wave blocks (a.fs_sgroups of the groups sweep)
+hipError_t onc_launch_forces_stencil(const onc_kargs& a, onc_u32 ngroups, hipStream_t s);
+// the same with 256-thread blocks (four independent force walkers per block)
+hipError_t onc_launch_forces_stencil4(const onc_kargs& a, onc_u32 ngroups, hipStream_t s);
 hipError_t onc_launch_rule_builtin(const onc_rule_args& r, int grid, hipStream_t s);
 hipError_t onc_launch_compact(const onc_kargs& a, int items, int grid, hipStream_t s);
 

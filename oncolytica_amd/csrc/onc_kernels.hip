@@ -583,7 +583,6 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // ---------------------------------------------------------------------------
 #define ONC_PAIR_LIST 16
 #define ONC_FORCES_BLOCK 64  // one wave per block
-#define ONC_FORCES_XCD_GROUP 8  // chunks of very uneven cost: interleave the XCDs
 
 // Candidates of the 27 Moore bins around (px, py, pz), z-plane by z-plane, in
 // template.wgsl:11-21 order (x fastest) and bin order inside a bin, skipping
@@ -597,7 +596,7 @@ template <class F, class G>
 __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
                                             onc_u32 self, const float4* __restrict__ pos4, onc_u32 (*s_rs)[ONC_FORCES_BLOCK],
                                             onc_u32 (*s_re)[ONC_FORCES_BLOCK], F pair, G batch_end) {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x & (ONC_FORCES_BLOCK - 1);  // lane: LDS slots are per wave
     const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
               vz = onc_vox1(pz, S.dx, S.dims[2]);
     const uint4* __restrict__ bins = (const uint4*)S.bins;
@@ -653,13 +652,23 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, fl
     }
 }
 
-__global__ __launch_bounds__(ONC_FORCES_BLOCK) void k_forces(onc_kargs a) {
-    // per-thread list of interacting neighbours, [slot][thread] (bank-conflict free)
-    __shared__ onc_u32 s_pairs[ONC_PAIR_LIST][ONC_FORCES_BLOCK];
-    // per-thread non-empty bin ranges of the current z-plane
-    __shared__ onc_u32 s_rs[9][ONC_FORCES_BLOCK], s_re[9][ONC_FORCES_BLOCK];
-    const int tid = threadIdx.x;
-    onc_for_cells<ONC_FORCES_XCD_GROUP>(a.cell_prefix, a.nsims, [&](int s, onc_u32 i) {
+// LDS of one forces wave: per-thread list of interacting neighbours
+// [slot][thread] (bank-conflict free) and the non-empty bin ranges of the
+// current z-plane.
+struct onc_forces_lds {
+    onc_u32 pairs[ONC_PAIR_LIST][ONC_FORCES_BLOCK];
+    onc_u32 rs[9][ONC_FORCES_BLOCK], re[9][ONC_FORCES_BLOCK];
+};
+
+// Forces of the cells of the chunks of one-wave block `blk` of `nblk`
+// (onc_for_chunks_xcd: blk % 8 is the block's XCD).
+__device__ __forceinline__ void onc_forces_wave(const onc_kargs& a, onc_u32 blk, onc_u32 nblk,
+                                                onc_forces_lds& L) {
+    onc_u32 (*s_pairs)[ONC_FORCES_BLOCK] = L.pairs;
+    onc_u32 (*s_rs)[ONC_FORCES_BLOCK] = L.rs;
+    onc_u32 (*s_re)[ONC_FORCES_BLOCK] = L.re;
+    const int tid = threadIdx.x & (ONC_FORCES_BLOCK - 1);  // lane: LDS slots are per wave
+    onc_for_chunks_xcd(a.cell_prefix, a.nsims, blk, nblk, [&](int s, onc_u32 i) {
         const onc_dsim& S = a.sims[s];
         const onc_u32 par = S.par;
         const onc_u32 epoch = S.step + 1u;
@@ -745,6 +754,11 @@ __global__ __launch_bounds__(ONC_FORCES_BLOCK) void k_forces(onc_kargs a) {
     });
 }
 
+__global__ __launch_bounds__(ONC_FORCES_BLOCK) void k_forces(onc_kargs a) {
+    __shared__ onc_forces_lds L;
+    onc_forces_wave(a, blockIdx.x, gridDim.x, L);
+}
+
 // ---------------------------------------------------------------------------
 // 7-point explicit sweep, all (sim, field) jobs in one launch. A block of
 // sbx x sby threads owns an xy tile and marches sbz planes along z, keeping
@@ -769,25 +783,33 @@ __device__ __forceinline__ float onc_sweep1(const onc_dfield& fd, float c, float
 
 // Register-blocked sweep for rows whose length is a multiple of 4: a thread
 // owns 4 consecutive voxels of a row (float4 loads / stores) and marches sbz
-// planes along z; rows y-1, y+1 and plane z+1 come in as float4, the two x
-// halo values as scalars. A 256-thread block tiles tw = min(X/4, sbx) float4
-// columns by 256 / tw rows (sbx is a power of two <= 256).
+// planes along z; rows y-1, y+1 and plane z+1 come in as float4. A block of
+// NT threads tiles tw = onc_stencil_tw(X/4, min(sbx, NT)) float4 columns by
+// NT / tw rows,
+// so each row segment of W = min(tw, 64) lanes holds consecutive float4 of
+// one row: the x halo values pass between neighbouring lanes by shuffle, and
+// only the segment's end lanes load them (one scalar load per segment
+// instead of two per lane: the scalar halo loads, not HBM, limited the sweep).
+template <int NT>
 __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_dfield& fd, int f, onc_u32 local,
                                                  int sbx, int sbz) {
     const int X = S.dims[0], Y = S.dims[1], Z = S.dims[2];
     const int X4 = X >> 2;
-    const int tw = X4 < sbx ? X4 : sbx;
-    const int th = 256 / tw;
+    const int tw = onc_stencil_tw(X4, sbx < NT ? sbx : NT);
+    const int th = NT / tw;
+    const int W = tw < 64 ? tw : 64;
     const int ntx = (X4 + tw - 1) / tw, nty = (Y + th - 1) / th;
     const int tx = (int)(local % (onc_u32)ntx);
     const onc_u32 rest = local / (onc_u32)ntx;
     const int ty = (int)(rest % (onc_u32)nty);
     const int tz = (int)(rest / (onc_u32)nty);
     const int t = (int)threadIdx.x;
-    if (t >= tw * th) return;
     const int x4 = tx * tw + t % tw;
     const int y = ty * th + t / tw;
+    // lanes past the row end or the last row leave; a remaining lane only
+    // takes a shuffled halo from a neighbour inside the row (hxm / hxp)
     if (x4 >= X4 || y >= Y) return;
+    const int sl = t % W;  // lane inside its row segment
     const int x0 = x4 * 4;
     const int z0 = tz * sbz;
     const int z1 = min(Z, z0 + sbz);
@@ -797,34 +819,39 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
     const size_t XY = (size_t)X * (size_t)Y;
     const size_t row = (size_t)x0 + (size_t)X * (size_t)y;
     const bool hxm = x0 > 0, hxp = x0 + 4 < X, hym = y > 0, hyp = y < Y - 1;
+    const bool ldl = hxm && sl == 0, ldr = hxp && sl == W - 1;
     const bool dir = fd.bc == 0, psrc = fd.bc == 1;
     size_t v = row + XY * (size_t)z0;
     float4 c = *(const float4*)(in + v);
     float4 cm = z0 > 0 ? *(const float4*)(in + v - XY) : c;
     float4 ym = hym ? *(const float4*)(in + v - (size_t)X) : c;
     float4 yp = hyp ? *(const float4*)(in + v + (size_t)X) : c;
-    float lx = hxm ? in[v - 1] : c.x;
-    float rx = hxp ? in[v + 4] : c.w;
+    float lx = ldl ? in[v - 1] : 0.0f;
+    float rx = ldr ? in[v + 4] : 0.0f;
     for (int z = z0; z < z1; z++, v += XY) {
         const bool nxt = z + 1 < Z;
         const bool more = z + 1 < z1;
         const size_t vn = v + XY;
         const float4 cp = nxt ? *(const float4*)(in + vn) : c;
         float4 nym = cp, nyp = cp;
-        float nlx = cp.x, nrx = cp.w;
+        float nlx = 0.0f, nrx = 0.0f;
         if (more) {
             if (hym) nym = *(const float4*)(in + vn - (size_t)X);
             if (hyp) nyp = *(const float4*)(in + vn + (size_t)X);
-            if (hxm) nlx = in[vn - 1];
-            if (hxp) nrx = in[vn + 4];
+            if (ldl) nlx = in[vn - 1];
+            if (ldr) nrx = in[vn + 4];
         }
+        const float sl_w = __shfl_up(c.w, 1, W);
+        const float sr_x = __shfl_down(c.x, 1, W);
+        const float hl = hxm ? (sl == 0 ? lx : sl_w) : c.x;
+        const float hr = hxp ? (sl == W - 1 ? rx : sr_x) : c.w;
         const float4 zm = z > 0 ? cm : c;
         const float4 zp = nxt ? cp : c;
         float4 o;
-        o.x = onc_sweep1(fd, c.x, lx, c.y, ym.x, yp.x, zm.x, zp.x);
+        o.x = onc_sweep1(fd, c.x, hl, c.y, ym.x, yp.x, zm.x, zp.x);
         o.y = onc_sweep1(fd, c.y, c.x, c.z, ym.y, yp.y, zm.y, zp.y);
         o.z = onc_sweep1(fd, c.z, c.y, c.w, ym.z, yp.z, zm.z, zp.z);
-        o.w = onc_sweep1(fd, c.w, c.z, rx, ym.w, yp.w, zm.w, zp.w);
+        o.w = onc_sweep1(fd, c.w, c.z, hr, ym.w, yp.w, zm.w, zp.w);
         if (dir) {
             if (z == 0 || z == Z - 1 || y == 0 || y == Y - 1) {
                 o = make_float4(fd.bc_value, fd.bc_value, fd.bc_value, fd.bc_value);
@@ -848,21 +875,25 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
     }
 }
 
-__global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
-    const onc_u32 b = onc_uniform(onc_xcd_block(blockIdx.x, gridDim.x));
+// One block of the sweep: logical block `b` of the (sim, field) job table,
+// NT threads. Rows that are not a multiple of 4 use bx x by scalar tiles with
+// bx = min(sbx, NT), by = NT / bx.
+template <int NT>
+__device__ __forceinline__ void onc_stencil_block(const onc_kargs& a, onc_u32 b) {
     const int jb = (int)onc_uniform((onc_u32)onc_find_seg(a.jobs_prefix, a.njobs, b));
     const onc_u32 job = onc_uniform(a.jobs[jb]);
     const onc_dsim& S = a.sims[job >> 8];
     if (S.frozen) return;
     const int f = (int)(job & 255u);
     const int X = S.dims[0], Y = S.dims[1], Z = S.dims[2];
-    const int sbx = a.sbx, sby = a.sby, sbz = a.sbz;
+    const int sbz = a.sbz;
     const onc_u32 local = b - a.jobs_prefix[jb];
     if ((X & 3) == 0) {
         const onc_dfield fdv = S.fields[f];
-        onc_stencil_vec4(S, fdv, f, local, sbx, sbz);
+        onc_stencil_vec4<NT>(S, fdv, f, local, a.sbx, sbz);
         return;
     }
+    const int sbx = a.sbx < NT ? a.sbx : NT, sby = NT / sbx;
     const onc_u32 nbx = (onc_u32)((X + sbx - 1) / sbx), nby = (onc_u32)((Y + sby - 1) / sby);
     const int bx = (int)(local % nbx);
     const onc_u32 rest = local / nbx;
@@ -921,6 +952,55 @@ __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
         xp = hxp ? nxp : cp;
         ym = hym ? nym : cp;
         yp = hyp ? nyp : cp;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
+    onc_stencil_block<256>(a, onc_uniform(onc_xcd_block(blockIdx.x, gridDim.x)));
+}
+
+// Forces and the sweep in one launch of one-wave blocks: neither reads what
+// the other writes (forces: packed positions -> position columns; sweep:
+// fields), and the force walk is bound by gather latency and ALU while the
+// sweep streams HBM, so running them side by side on the same CUs hides much
+// of one behind the other. Blocks go in groups of 8 (one per XCD), the sweep's
+// groups spread evenly among the force groups (a_.fs_sgroups of the
+// gridDim.x / 8 groups), so both parts keep their XCD-aware block orders and
+// proceed together through the dispatch order.
+__global__ __launch_bounds__(ONC_FORCES_BLOCK) void k_forces_stencil(onc_kargs a) {
+    __shared__ onc_forces_lds L;
+    const onc_u32 G = gridDim.x >> 3, Gs = a.fs_sgroups;
+    const onc_u32 pg = blockIdx.x >> 3, x = blockIdx.x & 7u;
+    const onc_u32 s0 = (onc_u32)(((unsigned long long)pg * Gs) / G);
+    const onc_u32 s1 = (onc_u32)(((unsigned long long)(pg + 1u) * Gs) / G);
+    if (s1 > s0) {
+        const onc_u32 ns = a.jobs_prefix[a.njobs];
+        const onc_u32 b = onc_xcd_block(s0 * 8u + x, Gs * 8u);
+        if (b < ns) onc_stencil_block<ONC_FORCES_BLOCK>(a, onc_uniform(b));
+    } else {
+        onc_forces_wave(a, (pg - s0) * 8u + x, (G - Gs) * 8u, L);
+    }
+}
+
+// The same with 256-thread blocks: sweep blocks keep k_stencil's tiling (four
+// rows share their y halos through L1), and the four waves of a force block
+// are independent walkers, each with its own LDS slots; wave w of force block
+// vb is walker 8 (4 floor(vb / 8) + w) + vb % 8, so walkers keep the block's
+// XCD and the four take adjacent chunks of it (similar density).
+__global__ __launch_bounds__(256) void k_forces_stencil4(onc_kargs a) {
+    __shared__ onc_forces_lds L[4];
+    const onc_u32 G = gridDim.x >> 3, Gs = a.fs_sgroups;
+    const onc_u32 pg = blockIdx.x >> 3, x = blockIdx.x & 7u;
+    const onc_u32 s0 = (onc_u32)(((unsigned long long)pg * Gs) / G);
+    const onc_u32 s1 = (onc_u32)(((unsigned long long)(pg + 1u) * Gs) / G);
+    if (s1 > s0) {
+        const onc_u32 ns = a.jobs_prefix[a.njobs];
+        const onc_u32 b = onc_xcd_block(s0 * 8u + x, Gs * 8u);
+        if (b < ns) onc_stencil_block<256>(a, onc_uniform(b));
+    } else {
+        const onc_u32 vb = (pg - s0) * 8u + x;
+        const onc_u32 w = threadIdx.x >> 6;
+        onc_forces_wave(a, onc_uniform(((vb >> 3) * 4u + w) * 8u + x), (G - Gs) * 32u, L[w]);
     }
 }
 
@@ -1265,6 +1345,14 @@ hipError_t onc_launch_permute_bins(const onc_kargs& a, int grid, hipStream_t s) 
 }
 hipError_t onc_launch_forces(const onc_kargs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_forces, dim3(grid), dim3(ONC_FORCES_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t onc_launch_forces_stencil(const onc_kargs& a, onc_u32 ngroups, hipStream_t s) {
+    hipLaunchKernelGGL(k_forces_stencil, dim3(8u * ngroups), dim3(ONC_FORCES_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t onc_launch_forces_stencil4(const onc_kargs& a, onc_u32 ngroups, hipStream_t s) {
+    hipLaunchKernelGGL(k_forces_stencil4, dim3(8u * ngroups), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 hipError_t onc_launch_stencil(const onc_kargs& a, onc_u32 nblocks, hipStream_t s) {

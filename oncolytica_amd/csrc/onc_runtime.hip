@@ -107,6 +107,9 @@ struct onc_batch {
     int items = 16;   // cells per thread of a sort tile
     int citems = 4;   // cells per thread of a compaction tile
     int sbx = 64, sby = 4, sbz = 32;
+    int fused = 1;    // forces and the sweep in one launch (k_forces_stencil)
+    int force_waves = 0;  // cap on the one-wave blocks of the force walk (0: one per 64 cells of capacity)
+    onc_u32 fs_groups = 0;  // groups of 8 blocks of that launch
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -628,17 +631,22 @@ static int batch_destroy_internal(onc_batch* b) {
     return ONC_OK;
 }
 
-// Blocks of one (sim, field) sweep; must match k_stencil's decomposition:
-// rows whose length is a multiple of 4 use the float4 tiling (min(X/4, sbx)
-// float4 columns by 256 / that many rows), others sbx x sby scalar tiles;
-// both march sbz planes.
-static onc_u32 stencil_blocks(int X, int Y, int Z, int sbx, int sby, int sbz) {
+static onc_u32 force_waves(onc_batch* b);
+
+// Blocks of one (sim, field) sweep with blocks of nt threads (256 for
+// k_stencil, 64 for k_forces_stencil); must match onc_stencil_block: rows
+// whose length is a multiple of 4 use the float4 tiling
+// (onc_stencil_tw(X/4, min(sbx, nt)) float4 columns by nt / that many rows),
+// others min(sbx, nt) x nt / min(sbx, nt) scalar tiles; both march sbz planes.
+static onc_u32 stencil_blocks(int X, int Y, int Z, int sbx, int sbz, int nt) {
     const int nz = (Z + sbz - 1) / sbz;
+    const int bx = sbx < nt ? sbx : nt;
     if ((X & 3) == 0) {
-        const int X4 = X >> 2, tw = X4 < sbx ? X4 : sbx, th = 256 / tw;
+        const int X4 = X >> 2, tw = onc_stencil_tw(X4, bx), th = nt / tw;
         return (onc_u32)(((X4 + tw - 1) / tw) * ((Y + th - 1) / th) * nz);
     }
-    return (onc_u32)(((X + sbx - 1) / sbx) * ((Y + sby - 1) / sby) * nz);
+    const int by = nt / bx;
+    return (onc_u32)(((X + bx - 1) / bx) * ((Y + by - 1) / by) * nz);
 }
 
 // (Re)build the device segment table, plan arrays and stencil job table.
@@ -661,7 +669,7 @@ static int batch_rebuild(onc_batch* b) {
         passes = std::max(passes, (s->key_bits + 7) / 8);
         total_cap += s->desc.capacity;
         const int X = s->desc.dims[0], Y = s->desc.dims[1], Z = s->desc.dims[2];
-        const onc_u32 nb = stencil_blocks(X, Y, Z, b->sbx, b->sby, b->sbz);
+        const onc_u32 nb = stencil_blocks(X, Y, Z, b->sbx, b->sbz, b->fused == 1 ? 64 : 256);
         for (int f = 0; f < s->desc.n_fields; f++) {
             jobs.push_back((onc_u32)(i << 8) | (onc_u32)f);
             jprefix.push_back(blocks);
@@ -736,6 +744,10 @@ static int batch_rebuild(onc_batch* b) {
     b->ra.cell_prefix = a.cell_prefix;
     b->stencil_blocks = blocks;
     b->total_cap = total_cap;
+    a.fs_sgroups = (blocks + 7u) / 8u;
+    // force groups of 8 blocks: one-wave blocks (fused 1), or four walkers per
+    // block (fused 2)
+    b->fs_groups = a.fs_sgroups + (b->fused == 2 ? (force_waves(b) + 31u) / 32u : force_waves(b) / 8u);
     b->passes = passes;
     b->dirty = false;
     return ONC_OK;
@@ -758,12 +770,17 @@ static int grid_cells(onc_batch* b) {
     uint64_t lim = (uint64_t)b->ctx->cus * 8;
     return (int)std::max<uint64_t>(1, std::min(need, lim));
 }
-// Forces: one-wave blocks, one per 64 cells of capacity (up to 65535), so
-// that each wave walks about one chunk and a slow chunk (dense bins) holds a
-// single wave slot while the dispatcher keeps the others busy.
-static int grid_forces(onc_batch* b) {
-    return (int)std::max<uint64_t>(1, std::min<uint64_t>((b->total_cap + 63) / 64, 65535));
+// Forces: one-wave blocks, by default one per 64 cells of capacity (up to
+// 65528), so that each wave walks about one chunk and a slow chunk (dense
+// bins) holds a single wave slot while the dispatcher keeps the others busy;
+// the force_waves tuning caps the count (each wave then strides through its
+// XCD's chunks). Always a multiple of 8 (onc_for_chunks_xcd).
+static onc_u32 force_waves(onc_batch* b) {
+    uint64_t n = std::min<uint64_t>((b->total_cap + 63) / 64, 65528);
+    if (b->force_waves > 0) n = std::min<uint64_t>(n, (uint64_t)b->force_waves);
+    return (onc_u32)((std::max<uint64_t>(n, 8) + 7) / 8 * 8);
 }
+static int grid_forces(onc_batch* b) { return (int)force_waves(b); }
 // Tile kernels: one block per tile the sims' capacity can need (blocks past
 // this step's live tiles leave at once), up to a cap beyond which blocks loop.
 static int grid_tiles(onc_batch* b) {
@@ -848,12 +865,19 @@ static int enqueue_step(onc_batch* b) {
     prof_mark(b, ONC_PH_PERMUTE_BINS, 0);
     CK(onc_launch_permute_bins(a, gc, st));
     prof_mark(b, ONC_PH_PERMUTE_BINS, 1);
-    prof_mark(b, ONC_PH_FORCES, 0);
-    CK(onc_launch_forces(a, grid_forces(b), st));
-    prof_mark(b, ONC_PH_FORCES, 1);
-    prof_mark(b, ONC_PH_STENCIL, 0);
-    CK(onc_launch_stencil(a, b->stencil_blocks, st));
-    prof_mark(b, ONC_PH_STENCIL, 1);
+    if (b->fused) {
+        prof_mark(b, ONC_PH_FORCES_STENCIL, 0);
+        if (b->fused == 2) CK(onc_launch_forces_stencil4(a, b->fs_groups, st));
+        else CK(onc_launch_forces_stencil(a, b->fs_groups, st));
+        prof_mark(b, ONC_PH_FORCES_STENCIL, 1);
+    } else {
+        prof_mark(b, ONC_PH_FORCES, 0);
+        CK(onc_launch_forces(a, grid_forces(b), st));
+        prof_mark(b, ONC_PH_FORCES, 1);
+        prof_mark(b, ONC_PH_STENCIL, 0);
+        CK(onc_launch_stencil(a, b->stencil_blocks, st));
+        prof_mark(b, ONC_PH_STENCIL, 1);
+    }
     prof_mark(b, ONC_PH_RULE, 0);
     CKR(launch_rule(b));
     prof_mark(b, ONC_PH_RULE, 1);
@@ -1021,6 +1045,27 @@ extern "C" int onc_batch_set_tuning(onc_batch* b, int items, int citems, int bx,
     b->sbx = bx;
     b->sby = by;
     b->sbz = bz;
+    b->dirty = true;
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_set_fused(onc_batch* b, int fused) {
+    if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
+    if (fused < 0 || fused > 2) return fail(ONC_ERR_INVALID, "fused must be 0, 1 or 2");
+    CK(hipSetDevice(b->ctx->device));
+    CK(hipStreamSynchronize(b->stream));
+    b->fused = fused;
+    b->dirty = true;
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_set_forces_waves(onc_batch* b, int waves) {
+    if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
+    if (waves < 0 || (waves > 0 && (waves < 8 || waves % 8 != 0)))
+        return fail(ONC_ERR_INVALID, "force waves must be 0 or a positive multiple of 8");
+    CK(hipSetDevice(b->ctx->device));
+    CK(hipStreamSynchronize(b->stream));
+    b->force_waves = waves;
     b->dirty = true;
     return ONC_OK;
 }
@@ -1209,7 +1254,7 @@ extern "C" int onc_stencil7(onc_ctx* c, const float* in, float* out, const int32
     ds.state = (onc_dstate*)(B + o_st);
     onc_dstate dst{};
     dst.stop = 0xFFFFFFFFu;
-    const onc_u32 nb = stencil_blocks(dims[0], dims[1], dims[2], sbx, sby, sbz);
+    const onc_u32 nb = stencil_blocks(dims[0], dims[1], dims[2], sbx, sbz, 256);
     onc_u32 jobs[4] = {0u, 0u, nb, 0u};  // jobs[0] = job 0, jobs_prefix = {0, nb}
     CK(hipMemcpyAsync(B + o_sim, &ds, sizeof(ds), hipMemcpyHostToDevice, c->stream));
     CK(hipMemcpyAsync(B + o_st, &dst, sizeof(dst), hipMemcpyHostToDevice, c->stream));

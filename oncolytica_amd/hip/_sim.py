@@ -145,6 +145,17 @@ class Batch:
     def set_tuning(self, items: int, citems: int, bx: int, by: int, bz: int) -> None:
         call("onc_batch_set_tuning", self.h, items, citems, bx, by, bz)
 
+    def set_fused(self, fused) -> None:
+        """Forces and the 7-point sweep in one launch (True / 1: one-wave
+        blocks; 2: 256-thread blocks) or in two (False / 0); results are
+        identical."""
+        call("onc_batch_set_fused", self.h, int(fused))
+
+    def set_forces_waves(self, waves: int) -> None:
+        """Cap on the one-wave blocks of the force walk (0: one per 64 cells
+        of capacity; else a positive multiple of 8); results are identical."""
+        call("onc_batch_set_forces_waves", self.h, waves)
+
     def step(self, n: int = 1, graph: bool = True) -> None:
         call("onc_batch_step", self.h, n, 1 if graph else 0)
 

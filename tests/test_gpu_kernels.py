@@ -8,7 +8,7 @@ import pytest
 
 from oracle import oracle as O
 from oncolytica_amd.hip import _abi
-from oncolytica_amd.hip._abi import BC_DIRICHLET, BC_POINT_SOURCE, FieldDesc, ptr
+from oncolytica_amd.hip._abi import BC_DIRICHLET, BC_NONE, BC_POINT_SOURCE, FieldDesc, ptr
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -128,6 +128,13 @@ def _fd(**kw):
     ((256, 256, 256), dict(rate=0.05, decay=0.001, bc=BC_POINT_SOURCE, bc_value=1.0, src_radius=16.0)),
     ((37, 91, 13), dict(rate=0.15, decay=0.01, bc=BC_POINT_SOURCE, bc_value=0.7, src_radius=5.0)),
     ((130, 66, 200), dict(rate=0.1, bc_value=0.25, lo=0.1, hi=0.9)),
+    # float4 rows whose X/4 is not a power of two (row segments padded to the
+    # next power of two, x halos shuffled between lanes): 48, 25, 129, 2, 1
+    ((192, 40, 24), dict(rate=0.1, bc_value=1.0)),
+    ((100, 30, 20), dict(rate=0.12, decay=0.002, bc=BC_POINT_SOURCE, bc_value=0.9, src_radius=9.0)),
+    ((516, 9, 7), dict(rate=0.1, bc=BC_NONE, lo=0.0, hi=1.0)),
+    ((8, 5, 6), dict(rate=0.1, bc_value=0.5)),
+    ((4, 3, 3), dict(rate=0.1, bc=BC_NONE)),
 ])
 def test_stencil_bit_exact(ctx, dims, kw):
     rng = np.random.default_rng(sum(dims))
