@@ -43,7 +43,8 @@ def algorithmic_bytes(phase: str, n_mean: float, nvox: int, n_fields: int, cell_
     permute 2S + key 4 B/cell + 8 B per occupied bin; forces 12 B read + 12 B
     written + 4 B key per cell; stencil 8 B/voxel/field; forces and stencil
     in one launch: the sum of the two; rule 2S + 4F + 4 B flags per cell;
-    compaction 4 B flag + 8 B scan + 2S per cell."""
+    compaction 4 B flag + 8 B scan + 2S per cell; the rule inside compaction
+    2S + 4F per cell (state and fields read, survivors written)."""
     S = cell_bytes
     return {
         "morton": 20.0 * n_mean,
@@ -52,6 +53,7 @@ def algorithmic_bytes(phase: str, n_mean: float, nvox: int, n_fields: int, cell_
         "forces": 28.0 * n_mean,
         "stencil": 8.0 * nvox * n_fields,
         "forces_stencil": 28.0 * n_mean + 8.0 * nvox * n_fields,
+        "rule_compact": (2 * S + 4 * n_fields) * n_mean,
         "rule": (2 * S + 4 * n_fields + 4) * n_mean,
         "compact": (4 + 8 + 2 * S) * n_mean,
     }.get(phase)
@@ -318,7 +320,7 @@ PROFILE_RUNS = {
 }
 KERNEL_PHASE = {"k_plan": "plan", "k_morton": "morton", "k_sort_pass": "sort", "k_permute_bins": "permute_bins",
                 "k_forces": "forces", "k_stencil": "stencil", "k_rule_tumor": "rule", "k_compact": "compact",
-                "k_forces_stencil": "forces_stencil", "k_copy_u32": "copy"}
+                "k_forces_stencil": "forces_stencil", "k_rule_compact": "rule_compact", "k_copy_u32": "copy"}
 COPY_BYTES = 4 * (64 << 20)  # one --calibrate copy launch reads and writes this many bytes
 
 

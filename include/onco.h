@@ -203,10 +203,15 @@ int onc_batch_set_fused(onc_batch* b, int fused);
    8, and each wave strides through its XCD's 64-cell chunks. Results are
    identical for every value. */
 int onc_batch_set_forces_waves(onc_batch* b, int waves);
+/* fused = 1 (the default): when every sim of the batch runs the built-in
+   tumour rule on the six-column tumour schema, the rule is evaluated inside
+   the compaction kernel (one launch; the rule's output never goes through
+   memory); 0: the rule kernel, then compaction. Results are identical. */
+int onc_batch_set_fused_rule(onc_batch* b, int fused);
 /* per-phase HIP-event timing of eager steps on the batch stream: bit k of
    phase_mask times phase k (0 plan, 1 Morton, 2 sort, 3 permute + bins,
-   4 forces, 5 stencil, 6 rule, 7 compaction, 8 forces + stencil in one launch;
-   9 is unused); 0 disables */
+   4 forces, 5 stencil, 6 rule, 7 compaction, 8 forces + stencil in one launch,
+   9 rule + compaction in one launch); 0 disables */
 int onc_batch_profile(onc_batch* b, int phase_mask);
 #define ONC_PHASES 10
 int onc_batch_phase_times(onc_batch* b, double* us_total, uint64_t* launches, int n);

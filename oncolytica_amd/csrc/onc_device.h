@@ -206,7 +206,10 @@ __device__ __forceinline__ void onc_rng_init(onc_rng& r, const onc_dsim& S, onc_
 
 __device__ __forceinline__ float onc_rand(onc_rng& r) {
     if ((r.c & 3u) == 0) onc_philox(r.id, r.step, r.c >> 2, 0u, r.k0, r.k1, r.buf);
-    onc_u32 w = r.buf[r.c & 3u];
+    // a select chain, not r.buf[r.c & 3]: a runtime index would put the
+    // buffer in scratch memory
+    const onc_u32 q = r.c & 3u;
+    const onc_u32 w = q == 0u ? r.buf[0] : (q == 1u ? r.buf[1] : (q == 2u ? r.buf[2] : r.buf[3]));
     r.c++;
     return (float)(w >> 8) * 5.9604644775390625e-08f;
 }

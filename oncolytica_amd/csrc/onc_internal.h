@@ -33,7 +33,7 @@ enum onc_phase {
     ONC_PH_RULE = 6,
     ONC_PH_COMPACT = 7,
     ONC_PH_FORCES_STENCIL = 8,
-    ONC_PH_OTHER = 9,
+    ONC_PH_RULE_COMPACT = 9,
 };
 
 
@@ -52,10 +52,10 @@ hipError_t onc_launch_forces(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_stencil(const onc_kargs& a, onc_u32 nblocks, hipStream_t s);
 // forces + sweep in one launch of 8 * ngroups one-wave blocks (a.fs_sgroups of the groups sweep)
 hipError_t onc_launch_forces_stencil(const onc_kargs& a, onc_u32 ngroups, hipStream_t s);
-// the same with 256-thread blocks (four independent force walkers per block)
-hipError_t onc_launch_forces_stencil4(const onc_kargs& a, onc_u32 ngroups, hipStream_t s);
 hipError_t onc_launch_rule_builtin(const onc_rule_args& r, int grid, hipStream_t s);
 hipError_t onc_launch_compact(const onc_kargs& a, int items, int grid, hipStream_t s);
+// built-in tumour rule fused into compaction (six-column tumour schema)
+hipError_t onc_launch_rule_compact(const onc_kargs& a, int items, int grid, hipStream_t s);
 
 // pure kernels
 hipError_t onc_launch_morton_keys(const float* px, const float* py, const float* pz, onc_u32 n, int X,

@@ -660,98 +660,104 @@ struct onc_forces_lds {
     onc_u32 rs[9][ONC_FORCES_BLOCK], re[9][ONC_FORCES_BLOCK];
 };
 
-// Forces of the cells of the chunks of one-wave block `blk` of `nblk`
-// (onc_for_chunks_xcd: blk % 8 is the block's XCD).
-__device__ __forceinline__ void onc_forces_wave(const onc_kargs& a, onc_u32 blk, onc_u32 nblk,
-                                                onc_forces_lds& L) {
+// Forces of cell i of sim s through global memory (bins table, packed
+// positions). Must be called by every lane of the wave that has a cell (the
+// list flush is wave-wide).
+__device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u32 i, onc_forces_lds& L) {
     onc_u32 (*s_pairs)[ONC_FORCES_BLOCK] = L.pairs;
     onc_u32 (*s_rs)[ONC_FORCES_BLOCK] = L.rs;
     onc_u32 (*s_re)[ONC_FORCES_BLOCK] = L.re;
     const int tid = threadIdx.x & (ONC_FORCES_BLOCK - 1);  // lane: LDS slots are per wave
-    onc_for_chunks_xcd(a.cell_prefix, a.nsims, blk, nblk, [&](int s, onc_u32 i) {
-        const onc_dsim& S = a.sims[s];
-        const onc_u32 par = S.par;
-        const onc_u32 epoch = S.step + 1u;
-        const float4* __restrict__ p4 = (const float4*)S.pos4;
-        const float4 me = p4[i];
-        const float px = me.x, py = me.y, pz = me.z;
-        const float R = S.R, Sc = 2.0f * S.cell_radius;
-        const float R2 = R * R, RmS = R - Sc, krep = S.k_rep, kadh = S.k_adh;
-        float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
-        // Phase B: the pair law over the listed neighbours, in visiting order.
-        onc_u32 npair = 0;
-        auto law = [&](const float4& o) {
-            const float ddx = px - o.x;
-            const float ddy = py - o.y;
-            const float ddz = pz - o.z;
-            float d2 = ddx * ddx + ddy * ddy;
-            d2 = d2 + ddz * ddz;
-            const float dist = __builtin_sqrtf(d2);
-            float f;
-            if (dist < Sc) {
-                f = krep * (Sc - dist);
-            } else {
-                if (kadh == 0.0f) return;
-                f = -(kadh * ((R - dist) / RmS));
-            }
-            const float sc = f / dist;
-            Fx = Fx + sc * ddx;
-            Fy = Fy + sc * ddy;
-            Fz = Fz + sc * ddz;
-        };
-        // the listed neighbours' positions are loaded four at a time, then
-        // accumulated in list order
-        auto flush = [&]() {
-            onc_u32 k = 0;
-            for (; k + 4u <= npair; k += 4u) {
-                float4 o[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) o[u] = p4[s_pairs[k + u][tid]];
-#pragma unroll
-                for (int u = 0; u < 4; u++) law(o[u]);
-            }
-            for (; k < npair; k++) law(p4[s_pairs[k][tid]]);
-            npair = 0;
-        };
-        // Phase A: cheap distance filter over the 27 bins; only pairs inside
-        // the interaction radius reach the expensive (sqrt, divide) path, so
-        // lanes diverge on short lists instead of on every candidate. A list
-        // holds at most 8 entries between batches (a batch adds at most 8);
-        // once any lane of the wave holds more, every lane still walking
-        // flushes together. A flush taken by one lane at a time would run
-        // the wave through the pair law once per lane.
-        forces_walk(S, epoch, px, py, pz, i, p4, s_rs, s_re,
-                    [&](const float4& o, onc_u32 j) {
-                        const float ddx = px - o.x;
-                        const float ddy = py - o.y;
-                        const float ddz = pz - o.z;
-                        float d2 = ddx * ddx + ddy * ddy;
-                        d2 = d2 + ddz * ddz;
-                        if (!(d2 < R2) || d2 == 0.0f) return;
-                        s_pairs[npair][tid] = j;
-                        npair++;
-                    },
-                    [&]() {
-                        if (__ballot(npair > ONC_PAIR_LIST / 2) != 0ull) flush();
-                    });
-        flush();
-        const float mob = S.mobility, md = S.max_disp;
-        float dpx = mob * Fx, dpy = mob * Fy, dpz = mob * Fz;
-        float l2 = dpx * dpx + dpy * dpy;
-        l2 = l2 + dpz * dpz;
-        if (l2 > md * md) {
-            float sc = md / __builtin_sqrtf(l2);
-            dpx = dpx * sc;
-            dpy = dpy * sc;
-            dpz = dpz * sc;
+    const onc_dsim& S = a.sims[s];
+    const onc_u32 par = S.par;
+    const onc_u32 epoch = S.step + 1u;
+    const float4* __restrict__ p4 = (const float4*)S.pos4;
+    const float4 me = p4[i];
+    const float px = me.x, py = me.y, pz = me.z;
+    const float R = S.R, Sc = 2.0f * S.cell_radius;
+    const float R2 = R * R, RmS = R - Sc, krep = S.k_rep, kadh = S.k_adh;
+    float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;
+    // Phase B: the pair law over the listed neighbours, in visiting order.
+    onc_u32 npair = 0;
+    auto law = [&](const float4& o) {
+        const float ddx = px - o.x;
+        const float ddy = py - o.y;
+        const float ddz = pz - o.z;
+        float d2 = ddx * ddx + ddy * ddy;
+        d2 = d2 + ddz * ddz;
+        const float dist = __builtin_sqrtf(d2);
+        float f;
+        if (dist < Sc) {
+            f = krep * (Sc - dist);
+        } else {
+            if (kadh == 0.0f) return;
+            f = -(kadh * ((R - dist) / RmS));
         }
-        float* cx = (float*)S.st[par][0];
-        float* cy = (float*)S.st[par][1];
-        float* cz = (float*)S.st[par][2];
-        cx[i] = onc_clampf(px + dpx, 0.0f, (float)S.dims[0] * S.dx);
-        cy[i] = onc_clampf(py + dpy, 0.0f, (float)S.dims[1] * S.dx);
-        cz[i] = onc_clampf(pz + dpz, 0.0f, (float)S.dims[2] * S.dx);
-    });
+        const float sc = f / dist;
+        Fx = Fx + sc * ddx;
+        Fy = Fy + sc * ddy;
+        Fz = Fz + sc * ddz;
+    };
+    // the listed neighbours' positions are loaded four at a time, then
+    // accumulated in list order
+    auto flush = [&]() {
+        onc_u32 k = 0;
+        for (; k + 4u <= npair; k += 4u) {
+            float4 o[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) o[u] = p4[s_pairs[k + u][tid]];
+#pragma unroll
+            for (int u = 0; u < 4; u++) law(o[u]);
+        }
+        for (; k < npair; k++) law(p4[s_pairs[k][tid]]);
+        npair = 0;
+    };
+    // Phase A: cheap distance filter over the 27 bins; only pairs inside
+    // the interaction radius reach the expensive (sqrt, divide) path, so
+    // lanes diverge on short lists instead of on every candidate. A list
+    // holds at most 8 entries between batches (a batch adds at most 8);
+    // once any lane of the wave holds more, every lane still walking
+    // flushes together. A flush taken by one lane at a time would run
+    // the wave through the pair law once per lane.
+    forces_walk(S, epoch, px, py, pz, i, p4, s_rs, s_re,
+                [&](const float4& o, onc_u32 j) {
+                    const float ddx = px - o.x;
+                    const float ddy = py - o.y;
+                    const float ddz = pz - o.z;
+                    float d2 = ddx * ddx + ddy * ddy;
+                    d2 = d2 + ddz * ddz;
+                    if (!(d2 < R2) || d2 == 0.0f) return;
+                    s_pairs[npair][tid] = j;
+                    npair++;
+                },
+                [&]() {
+                    if (__ballot(npair > ONC_PAIR_LIST / 2) != 0ull) flush();
+                });
+    flush();
+    const float mob = S.mobility, md = S.max_disp;
+    float dpx = mob * Fx, dpy = mob * Fy, dpz = mob * Fz;
+    float l2 = dpx * dpx + dpy * dpy;
+    l2 = l2 + dpz * dpz;
+    if (l2 > md * md) {
+        float sc = md / __builtin_sqrtf(l2);
+        dpx = dpx * sc;
+        dpy = dpy * sc;
+        dpz = dpz * sc;
+    }
+    float* cx = (float*)S.st[par][0];
+    float* cy = (float*)S.st[par][1];
+    float* cz = (float*)S.st[par][2];
+    cx[i] = onc_clampf(px + dpx, 0.0f, (float)S.dims[0] * S.dx);
+    cy[i] = onc_clampf(py + dpy, 0.0f, (float)S.dims[1] * S.dx);
+    cz[i] = onc_clampf(pz + dpz, 0.0f, (float)S.dims[2] * S.dx);
+}
+
+// Forces of the cells of the chunks of one-wave block `blk` of `nblk`
+// (onc_for_chunks_xcd: blk % 8 is the block's XCD).
+__device__ __forceinline__ void onc_forces_wave(const onc_kargs& a, onc_u32 blk, onc_u32 nblk,
+                                                onc_forces_lds& L) {
+    onc_for_chunks_xcd(a.cell_prefix, a.nsims, blk, nblk,
+                       [&](int s, onc_u32 i) __attribute__((always_inline)) { onc_forces_cell(a, s, i, L); });
 }
 
 __global__ __launch_bounds__(ONC_FORCES_BLOCK) void k_forces(onc_kargs a) {
@@ -982,82 +988,85 @@ __global__ __launch_bounds__(ONC_FORCES_BLOCK) void k_forces_stencil(onc_kargs a
     }
 }
 
-// The same with 256-thread blocks: sweep blocks keep k_stencil's tiling (four
-// rows share their y halos through L1), and the four waves of a force block
-// are independent walkers, each with its own LDS slots; wave w of force block
-// vb is walker 8 (4 floor(vb / 8) + w) + vb % 8, so walkers keep the block's
-// XCD and the four take adjacent chunks of it (similar density).
-__global__ __launch_bounds__(256) void k_forces_stencil4(onc_kargs a) {
-    __shared__ onc_forces_lds L[4];
-    const onc_u32 G = gridDim.x >> 3, Gs = a.fs_sgroups;
-    const onc_u32 pg = blockIdx.x >> 3, x = blockIdx.x & 7u;
-    const onc_u32 s0 = (onc_u32)(((unsigned long long)pg * Gs) / G);
-    const onc_u32 s1 = (onc_u32)(((unsigned long long)(pg + 1u) * Gs) / G);
-    if (s1 > s0) {
-        const onc_u32 ns = a.jobs_prefix[a.njobs];
-        const onc_u32 b = onc_xcd_block(s0 * 8u + x, Gs * 8u);
-        if (b < ns) onc_stencil_block<256>(a, onc_uniform(b));
-    } else {
-        const onc_u32 vb = (pg - s0) * 8u + x;
-        const onc_u32 w = threadIdx.x >> 6;
-        onc_forces_wave(a, onc_uniform(((vb >> 3) * 4u + w) * 8u + x), (G - Gs) * 32u, L[w]);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Built-in tumour rule (the BASELINE model). Reads the permuted state (other
 // buffer), the integrated position and the swept fields; writes the cell's
 // columns to the current buffer, its alive/divide flags and its daughter.
 // ---------------------------------------------------------------------------
+// The rule for one cell i of sim S: reads its integrated position (current
+// buffer), id / age / generation (permuted buffer) and the swept fields; gives
+// its alive / divide decisions, its new age and generation, and where a
+// daughter would go.
+struct onc_tumor_out {
+    float px, py, pz, age, dpx, dpy, dpz;
+    onc_u32 id, gen, alive, div;
+};
+
+__device__ __forceinline__ onc_tumor_out onc_tumor_rule(const onc_dsim& S, onc_u32 i) {
+    const onc_u32 par = S.par;
+    onc_gcol const* T = S.st[par ^ 1u];
+    onc_gcol const* C = S.st[par];
+    onc_tumor_out o;
+    o.px = ((const float*)C[0])[i];
+    o.py = ((const float*)C[1])[i];
+    o.pz = ((const float*)C[2])[i];
+    o.id = T[3][i];
+    float age = __uint_as_float(T[4][i]);
+    o.gen = T[5][i];
+    const onc_u32 v = onc_field_index(S, o.px, o.py, o.pz);
+    const float o2 = S.fld[par ^ 1u][0][v];
+    onc_rng rng;
+    onc_rng_init(rng, S, o.id, S.step);
+    age = age + 1.0f;
+    o.alive = 1u;
+    o.div = 0u;
+    o.dpx = 0.0f;
+    o.dpy = 0.0f;
+    o.dpz = 0.0f;
+    if (o2 < S.params[2]) {
+        o.alive = 0u;
+    } else {
+        if (S.n_fields > 1 && S.params[3] > 0.0f) {
+            const float drug = S.fld[par ^ 1u][1][v];
+            const float u = onc_rand(rng);
+            if (u < S.params[3] * drug) o.alive = 0u;
+        }
+        if (o.alive && age > S.params[0] && o2 > S.params[1]) {
+            float ux, uy, uz;
+            onc_rand_dir(rng, &ux, &uy, &uz);
+            const float ofs = S.params[4];
+            o.dpx = onc_clampf(o.px + ux * ofs, 0.0f, (float)S.dims[0] * S.dx);
+            o.dpy = onc_clampf(o.py + uy * ofs, 0.0f, (float)S.dims[1] * S.dx);
+            o.dpz = onc_clampf(o.pz + uz * ofs, 0.0f, (float)S.dims[2] * S.dx);
+            age = 0.0f;
+            o.gen = o.gen + 1u;
+            o.div = 1u;
+        }
+    }
+    o.age = age;
+    return o;
+}
+
 __global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
     onc_for_cells(r.cell_prefix, r.nsims, [&](int s, onc_u32 i) {
         const onc_dsim& S = r.sims[s];
         const onc_u32 par = S.par;
         onc_gcol const* T = S.st[par ^ 1u];
         onc_gcol const* C = S.st[par];
-        float px = ((const float*)C[0])[i], py = ((const float*)C[1])[i], pz = ((const float*)C[2])[i];
-        const onc_u32 id = T[3][i];
-        float age = __uint_as_float(T[4][i]);
-        onc_u32 gen = T[5][i];
-        const onc_u32 v = onc_field_index(S, px, py, pz);
-        const float o2 = S.fld[par ^ 1u][0][v];
-        onc_rng rng;
-        onc_rng_init(rng, S, id, S.step);
-        age = age + 1.0f;
-        onc_u32 alive = 1u, div = 0u;
-        float dpx = 0.0f, dpy = 0.0f, dpz = 0.0f;
-        if (o2 < S.params[2]) {
-            alive = 0u;
-        } else {
-            if (S.n_fields > 1 && S.params[3] > 0.0f) {
-                const float drug = S.fld[par ^ 1u][1][v];
-                const float u = onc_rand(rng);
-                if (u < S.params[3] * drug) alive = 0u;
-            }
-            if (alive && age > S.params[0] && o2 > S.params[1]) {
-                float ux, uy, uz;
-                onc_rand_dir(rng, &ux, &uy, &uz);
-                const float ofs = S.params[4];
-                dpx = onc_clampf(px + ux * ofs, 0.0f, (float)S.dims[0] * S.dx);
-                dpy = onc_clampf(py + uy * ofs, 0.0f, (float)S.dims[1] * S.dx);
-                dpz = onc_clampf(pz + uz * ofs, 0.0f, (float)S.dims[2] * S.dx);
-                age = 0.0f;
-                gen = gen + 1u;
-                div = 1u;
-            }
-        }
-        C[3][i] = id;
-        C[4][i] = __float_as_uint(age);
-        C[5][i] = gen;
+        const onc_tumor_out o = onc_tumor_rule(S, i);
+        C[3][i] = o.id;
+        C[4][i] = __float_as_uint(o.age);
+        C[5][i] = o.gen;
         for (int c = 6; c < S.n_cols; c++) C[c][i] = T[c][i];
-        S.flags[i] = alive | (div << 1);
-        if (div) {
-            S.dau[0][i] = __float_as_uint(dpx);
-            S.dau[1][i] = __float_as_uint(dpy);
-            S.dau[2][i] = __float_as_uint(dpz);
+        S.flags[i] = o.alive | (o.div << 1);
+        if (o.div) {
+            S.dau[0][i] = __float_as_uint(o.dpx);
+            S.dau[1][i] = __float_as_uint(o.dpy);
+            S.dau[2][i] = __float_as_uint(o.dpz);
             S.dau[3][i] = 0u;
             S.dau[4][i] = __float_as_uint(0.0f);
-            S.dau[5][i] = gen;
+            S.dau[5][i] = o.gen;
             for (int c = 6; c < S.n_cols; c++) S.dau[c][i] = T[c][i];
         }
     });
@@ -1095,6 +1104,64 @@ __device__ __forceinline__ void compact_copy(onc_gcol const* src, onc_gcol const
     }
 }
 
+// Survivor slots and daughter ranks of one compaction tile: on entry
+// s_aslot[j] holds the flags (bit 0 alive, bit 1 divides) of cell base + j of
+// tile t (0 past the tile's cells); on exit s_aslot[j] / s_dslot[j] hold the
+// cell's slot among the sim's survivors / its rank among the sim's daughters,
+// or NONE. Scan of both counts with decoupled look-back; the sim's last tile
+// records the totals. All threads of the block call it.
+template <int ITEMS>
+__device__ __forceinline__ void compact_slots(const onc_kargs& a, onc_u32 g, const onc_dsim& S, onc_u32 t,
+                                              onc_u32 ntiles_s, onc_u32* s_aslot, onc_u32* s_dslot, onc_u32* s_tmp,
+                                              onc_u32& s_pa, onc_u32& s_pd) {
+    constexpr onc_u32 NONE = 0xFFFFFFFFu;
+    const int tid = threadIdx.x;
+    __syncthreads();
+    onc_u32 la = 0u, ldv = 0u;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        onc_u32 f = s_aslot[tid * ITEMS + k];
+        la += f & 1u;
+        ldv += (f >> 1) & 1u;
+    }
+    onc_u32 tp;
+    const onc_u32 ex = block_exscan256(la | (ldv << 16), s_tmp, &tp);
+    const onc_u32 ta = tp & 0xFFFFu, td = tp >> 16;
+    if (tid < 64) {
+        onc_u32 pa = 0u, pd = 0u;
+        onc_u64* mine = &a.comp_status[g];
+        const onc_u64 agg = (onc_u64)ta | ((onc_u64)td << 31);
+        if (t == 0u) {
+            if (tid == 0) st_relaxed64(mine, ONC_F64_P | agg);
+        } else {
+            if (tid == 0) st_relaxed64(mine, ONC_F64_A | agg);
+            lookback_wave64(a.comp_status, g, g - t, true, a.err, 2u, &pa, &pd);
+            if (tid == 0) st_relaxed64(mine, ONC_F64_P | ((onc_u64)(pa + ta) | ((onc_u64)(pd + td) << 31)));
+        }
+        if (tid == 0) {
+            s_pa = pa;
+            s_pd = pd;
+            if (t + 1u == ntiles_s) {
+                S.state->nsurv = pa + ta;
+                S.state->ndiv = pd + td;
+            }
+        }
+    }
+    __syncthreads();
+    onc_u32 ra = s_pa + (ex & 0xFFFFu), rd = s_pd + (ex >> 16);
+    onc_u32 fl[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) fl[k] = s_aslot[tid * ITEMS + k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const onc_u32 f = fl[k];
+        s_aslot[tid * ITEMS + k] = (f & 1u) ? ra++ : NONE;
+        s_dslot[tid * ITEMS + k] = (f & 2u) ? rd++ : NONE;
+    }
+    __syncthreads();
+}
+
 template <int ITEMS>
 __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
     constexpr int TILE = 256 * ITEMS;
@@ -1120,50 +1187,7 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
             onc_u32 j = (onc_u32)(k * 256 + tid);
             s_aslot[j] = j < cnt ? S.flags[base + j] : 0u;
         }
-        __syncthreads();
-        onc_u32 la = 0u, ldv = 0u;
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            onc_u32 f = s_aslot[tid * ITEMS + k];
-            la += f & 1u;
-            ldv += (f >> 1) & 1u;
-        }
-        onc_u32 tp;
-        const onc_u32 ex = block_exscan256(la | (ldv << 16), s_tmp, &tp);
-        const onc_u32 ta = tp & 0xFFFFu, td = tp >> 16;
-        if (tid < 64) {
-            onc_u32 pa = 0u, pd = 0u;
-            onc_u64* mine = &a.comp_status[g];
-            const onc_u64 agg = (onc_u64)ta | ((onc_u64)td << 31);
-            if (t == 0u) {
-                if (tid == 0) st_relaxed64(mine, ONC_F64_P | agg);
-            } else {
-                if (tid == 0) st_relaxed64(mine, ONC_F64_A | agg);
-                lookback_wave64(a.comp_status, g, g - t, true, a.err, 2u, &pa, &pd);
-                if (tid == 0) st_relaxed64(mine, ONC_F64_P | ((onc_u64)(pa + ta) | ((onc_u64)(pd + td) << 31)));
-            }
-            if (tid == 0) {
-                s_pa = pa;
-                s_pd = pd;
-                if (t + 1u == ntiles_s) {
-                    S.state->nsurv = pa + ta;
-                    S.state->ndiv = pd + td;
-                }
-            }
-        }
-        __syncthreads();
-        onc_u32 ra = s_pa + (ex & 0xFFFFu), rd = s_pd + (ex >> 16);
-        onc_u32 fl[ITEMS];
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++) fl[k] = s_aslot[tid * ITEMS + k];
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            const onc_u32 f = fl[k];
-            s_aslot[tid * ITEMS + k] = (f & 1u) ? ra++ : NONE;
-            s_dslot[tid * ITEMS + k] = (f & 2u) ? rd++ : NONE;
-        }
-        __syncthreads();
+        compact_slots<ITEMS>(a, g, S, t, ntiles_s, s_aslot, s_dslot, s_tmp, s_pa, s_pd);
         // Copies in groups of four columns: all loads of a group are issued
         // before any of its stores (a store to one column may alias a later
         // column's load, so the compiler keeps program order and each group
@@ -1184,6 +1208,70 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
 #pragma unroll
             for (int k = 0; k < ITEMS; k++)
                 if (sd[k] != NONE) S.dau2[ONC_COL_ID_DEV][sd[k]] = S.next_id + sd[k];
+        }
+        __syncthreads();
+    }
+}
+
+// The built-in tumour rule fused into compaction, for the six-column tumour
+// schema (pos.x, pos.y, pos.z, id, age, generation): each tile evaluates the
+// rule for its cells into registers, scans the flags, and writes survivors
+// and daughters straight to their slots, so neither the flags nor the rule's
+// output columns nor the sparse daughters make a round trip through memory.
+// Survivors go to the permuted buffer in place: a tile writes only after its
+// look-back, i.e. after every earlier tile of its sim has read its cells, and
+// its slots lie below the end of its own cells, which it holds in registers.
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_rule_compact(onc_kargs a) {
+    constexpr int TILE = 256 * ITEMS;
+    constexpr onc_u32 NONE = 0xFFFFFFFFu;
+    __shared__ onc_u32 s_aslot[TILE];
+    __shared__ onc_u32 s_dslot[TILE];
+    __shared__ onc_u32 s_tmp[8];
+    __shared__ onc_u32 s_pa, s_pd;
+    const int tid = threadIdx.x;
+    const onc_u32 total = a.ctile_prefix[a.nsims];
+    for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {  // static tile order, as in k_sort_pass
+        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.ctile_prefix, a.nsims, g));
+        const onc_dsim& S = a.sims[s];
+        const onc_u32 t = g - a.ctile_prefix[s];
+        const onc_u32 ntiles_s = a.ctile_prefix[s + 1] - a.ctile_prefix[s];
+        const onc_u32 n = S.n_live;
+        const onc_u32 base = t * (onc_u32)TILE;
+        const onc_u32 cnt = min((onc_u32)TILE, n - base);
+        const onc_u32 par = S.par;
+        onc_tumor_out o[ITEMS];
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const onc_u32 j = (onc_u32)(k * 256 + tid);
+            o[k].alive = 0u;
+            o[k].div = 0u;
+            if (j < cnt) o[k] = onc_tumor_rule(S, base + j);
+            s_aslot[j] = o[k].alive | (o[k].div << 1);
+        }
+        compact_slots<ITEMS>(a, g, S, t, ntiles_s, s_aslot, s_dslot, s_tmp, s_pa, s_pd);
+        onc_gcol const* D = S.st[par ^ 1u];
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const onc_u32 j = (onc_u32)(k * 256 + tid);
+            if (j >= cnt) continue;
+            const onc_u32 sa = s_aslot[j], sd = s_dslot[j];
+            if (sa != NONE) {
+                D[0][sa] = __float_as_uint(o[k].px);
+                D[1][sa] = __float_as_uint(o[k].py);
+                D[2][sa] = __float_as_uint(o[k].pz);
+                D[3][sa] = o[k].id;
+                D[4][sa] = __float_as_uint(o[k].age);
+                D[5][sa] = o[k].gen;
+            }
+            if (sd != NONE) {
+                S.dau2[0][sd] = __float_as_uint(o[k].dpx);
+                S.dau2[1][sd] = __float_as_uint(o[k].dpy);
+                S.dau2[2][sd] = __float_as_uint(o[k].dpz);
+                S.dau2[3][sd] = S.next_id + sd;
+                S.dau2[4][sd] = __float_as_uint(0.0f);
+                S.dau2[5][sd] = o[k].gen;
+            }
         }
         __syncthreads();
     }
@@ -1351,10 +1439,6 @@ hipError_t onc_launch_forces_stencil(const onc_kargs& a, onc_u32 ngroups, hipStr
     hipLaunchKernelGGL(k_forces_stencil, dim3(8u * ngroups), dim3(ONC_FORCES_BLOCK), 0, s, a);
     return hipGetLastError();
 }
-hipError_t onc_launch_forces_stencil4(const onc_kargs& a, onc_u32 ngroups, hipStream_t s) {
-    hipLaunchKernelGGL(k_forces_stencil4, dim3(8u * ngroups), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
 hipError_t onc_launch_stencil(const onc_kargs& a, onc_u32 nblocks, hipStream_t s) {
     if (nblocks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_stencil, dim3(nblocks), dim3(256), 0, s, a);
@@ -1366,6 +1450,10 @@ hipError_t onc_launch_rule_builtin(const onc_rule_args& r, int grid, hipStream_t
 }
 hipError_t onc_launch_compact(const onc_kargs& a, int items, int grid, hipStream_t s) {
     ONC_DISPATCH_ITEMS(items, k_compact, grid, s, a);
+    return hipGetLastError();
+}
+hipError_t onc_launch_rule_compact(const onc_kargs& a, int items, int grid, hipStream_t s) {
+    ONC_DISPATCH_ITEMS(items, k_rule_compact, grid, s, a);
     return hipGetLastError();
 }
 

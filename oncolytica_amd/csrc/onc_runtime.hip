@@ -104,11 +104,13 @@ struct onc_batch {
     uint64_t total_cap = 0;
     int passes = 1;
     // tuning
-    int items = 16;   // cells per thread of a sort tile
+    int items = 8;    // cells per thread of a sort tile
     int citems = 4;   // cells per thread of a compaction tile
-    int sbx = 64, sby = 4, sbz = 32;
+    int sbx = 64, sby = 4, sbz = 16;
     int fused = 1;    // forces and the sweep in one launch (k_forces_stencil)
     int force_waves = 0;  // cap on the one-wave blocks of the force walk (0: one per 64 cells of capacity)
+    int fuse_rule = 1;    // built-in rule fused into compaction where the schema allows
+    bool rule_compact = false;  // this batch runs k_rule_compact (set by batch_rebuild)
     onc_u32 fs_groups = 0;  // groups of 8 blocks of that launch
     // graph
     hipGraph_t graph = nullptr;
@@ -669,7 +671,7 @@ static int batch_rebuild(onc_batch* b) {
         passes = std::max(passes, (s->key_bits + 7) / 8);
         total_cap += s->desc.capacity;
         const int X = s->desc.dims[0], Y = s->desc.dims[1], Z = s->desc.dims[2];
-        const onc_u32 nb = stencil_blocks(X, Y, Z, b->sbx, b->sbz, b->fused == 1 ? 64 : 256);
+        const onc_u32 nb = stencil_blocks(X, Y, Z, b->sbx, b->sbz, b->fused ? 64 : 256);
         for (int f = 0; f < s->desc.n_fields; f++) {
             jobs.push_back((onc_u32)(i << 8) | (onc_u32)f);
             jprefix.push_back(blocks);
@@ -744,10 +746,14 @@ static int batch_rebuild(onc_batch* b) {
     b->ra.cell_prefix = a.cell_prefix;
     b->stencil_blocks = blocks;
     b->total_cap = total_cap;
+    // the fused rule + compaction serves the built-in rule on the six-column
+    // tumour schema (every other column layout goes through k_rule_tumor,
+    // which copies extra columns, and k_compact)
+    b->rule_compact = b->fuse_rule != 0 && S > 0;
+    for (size_t i = 0; i < S; i++)
+        if (b->sims[i]->rule_fn || b->sims[i]->desc.n_cols != 6) b->rule_compact = false;
     a.fs_sgroups = (blocks + 7u) / 8u;
-    // force groups of 8 blocks: one-wave blocks (fused 1), or four walkers per
-    // block (fused 2)
-    b->fs_groups = a.fs_sgroups + (b->fused == 2 ? (force_waves(b) + 31u) / 32u : force_waves(b) / 8u);
+    b->fs_groups = a.fs_sgroups + force_waves(b) / 8u;  // groups of 8 one-wave blocks
     b->passes = passes;
     b->dirty = false;
     return ONC_OK;
@@ -867,8 +873,7 @@ static int enqueue_step(onc_batch* b) {
     prof_mark(b, ONC_PH_PERMUTE_BINS, 1);
     if (b->fused) {
         prof_mark(b, ONC_PH_FORCES_STENCIL, 0);
-        if (b->fused == 2) CK(onc_launch_forces_stencil4(a, b->fs_groups, st));
-        else CK(onc_launch_forces_stencil(a, b->fs_groups, st));
+        CK(onc_launch_forces_stencil(a, b->fs_groups, st));
         prof_mark(b, ONC_PH_FORCES_STENCIL, 1);
     } else {
         prof_mark(b, ONC_PH_FORCES, 0);
@@ -878,12 +883,18 @@ static int enqueue_step(onc_batch* b) {
         CK(onc_launch_stencil(a, b->stencil_blocks, st));
         prof_mark(b, ONC_PH_STENCIL, 1);
     }
-    prof_mark(b, ONC_PH_RULE, 0);
-    CKR(launch_rule(b));
-    prof_mark(b, ONC_PH_RULE, 1);
-    prof_mark(b, ONC_PH_COMPACT, 0);
-    CK(onc_launch_compact(a, b->citems, grid_ctiles(b), st));
-    prof_mark(b, ONC_PH_COMPACT, 1);
+    if (b->rule_compact) {
+        prof_mark(b, ONC_PH_RULE_COMPACT, 0);
+        CK(onc_launch_rule_compact(a, b->citems, grid_ctiles(b), st));
+        prof_mark(b, ONC_PH_RULE_COMPACT, 1);
+    } else {
+        prof_mark(b, ONC_PH_RULE, 0);
+        CKR(launch_rule(b));
+        prof_mark(b, ONC_PH_RULE, 1);
+        prof_mark(b, ONC_PH_COMPACT, 0);
+        CK(onc_launch_compact(a, b->citems, grid_ctiles(b), st));
+        prof_mark(b, ONC_PH_COMPACT, 1);
+    }
     return ONC_OK;
 }
 
@@ -1051,10 +1062,20 @@ extern "C" int onc_batch_set_tuning(onc_batch* b, int items, int citems, int bx,
 
 extern "C" int onc_batch_set_fused(onc_batch* b, int fused) {
     if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
-    if (fused < 0 || fused > 2) return fail(ONC_ERR_INVALID, "fused must be 0, 1 or 2");
+    if (fused != 0 && fused != 1) return fail(ONC_ERR_INVALID, "fused must be 0 or 1");
     CK(hipSetDevice(b->ctx->device));
     CK(hipStreamSynchronize(b->stream));
     b->fused = fused;
+    b->dirty = true;
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_set_fused_rule(onc_batch* b, int fused) {
+    if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
+    if (fused != 0 && fused != 1) return fail(ONC_ERR_INVALID, "fused must be 0 or 1");
+    CK(hipSetDevice(b->ctx->device));
+    CK(hipStreamSynchronize(b->stream));
+    b->fuse_rule = fused;
     b->dirty = true;
     return ONC_OK;
 }

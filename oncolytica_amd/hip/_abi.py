@@ -38,7 +38,7 @@ COL_I32 = 1
 COL_U32 = 2
 
 PHASE_NAMES = ["plan", "morton", "sort", "permute_bins", "forces", "stencil", "rule", "compact",
-               "forces_stencil", "other"]
+               "forces_stencil", "rule_compact"]
 
 
 class OncError(RuntimeError):
@@ -117,6 +117,7 @@ _PROTOS = {
     "onc_batch_sync": (C.c_int, [_P]),
     "onc_batch_set_tuning": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "onc_batch_set_fused": (C.c_int, [_P, C.c_int]),
+    "onc_batch_set_fused_rule": (C.c_int, [_P, C.c_int]),
     "onc_batch_set_forces_waves": (C.c_int, [_P, C.c_int]),
     "onc_batch_profile": (C.c_int, [_P, C.c_int]),
     "onc_batch_phase_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_int]),

@@ -41,11 +41,11 @@ _memo: dict[str, "Tuning"] = {}
 
 @dataclass(frozen=True)
 class Tuning:
-    items: int = 16
+    items: int = 8
     citems: int = 4
     sbx: int = 64
     sby: int = 4
-    sbz: int = 32
+    sbz: int = 16
 
     def apply(self, batch: Batch) -> None:
         batch.set_tuning(self.items, self.citems, self.sbx, self.sby, self.sbz)

@@ -145,11 +145,16 @@ class Batch:
     def set_tuning(self, items: int, citems: int, bx: int, by: int, bz: int) -> None:
         call("onc_batch_set_tuning", self.h, items, citems, bx, by, bz)
 
-    def set_fused(self, fused) -> None:
-        """Forces and the 7-point sweep in one launch (True / 1: one-wave
-        blocks; 2: 256-thread blocks) or in two (False / 0); results are
-        identical."""
-        call("onc_batch_set_fused", self.h, int(fused))
+    def set_fused(self, fused: bool) -> None:
+        """Forces and the 7-point sweep in one launch (True, the default) or
+        in two (False); results are identical."""
+        call("onc_batch_set_fused", self.h, 1 if fused else 0)
+
+    def set_fused_rule(self, fused: bool) -> None:
+        """The built-in tumour rule inside the compaction kernel (True, the
+        default, where the schema allows) or as its own launch (False);
+        results are identical."""
+        call("onc_batch_set_fused_rule", self.h, 1 if fused else 0)
 
     def set_forces_waves(self, waves: int) -> None:
         """Cap on the one-wave blocks of the force walk (0: one per 64 cells
@@ -168,7 +173,7 @@ class Batch:
         if not enable:
             mask = 0
         elif phases is None:
-            mask = (1 << 9) - 1
+            mask = (1 << _abi.PHASES) - 1
         else:
             mask = 0
             for p in phases:

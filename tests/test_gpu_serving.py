@@ -30,12 +30,13 @@ def _small_ensemble():
     return ws + [odd]
 
 
-def _run_batch(ctx, ws, tuning, steps, fused=1, forces_waves=0):
+def _run_batch(ctx, ws, tuning, steps, fused=1, forces_waves=0, fused_rule=True):
     sims = [Sim(ctx, w.desc()) for w in ws]
     b = Batch(ctx)
     if tuning:
         tuning.apply(b)
     b.set_fused(fused)
+    b.set_fused_rule(fused_rule)
     if forces_waves:
         b.set_forces_waves(forces_waves)
     for s, w in zip(sims, ws):
@@ -70,20 +71,32 @@ def test_autotuner_returns_a_valid_shape_and_caches(ctx, tmp_path):
     assert tune(ctx, members, steps=1, cache_file=cache) == t
 
 
-@pytest.mark.parametrize("fused,tuning", [(1, None), (2, None), (1, Tuning(8, 8, 16, 16, 8)),
-                                          (2, Tuning(16, 4, 256, 1, 64)), (2, Tuning(4, 16, 32, 8, 8))])
-def test_fused_forces_stencil_equals_separate_launches(ctx, fused, tuning):
-    """Forces + sweep in one launch (one-wave blocks, or 256-thread blocks
-    with four force walkers each) gives the bits of the two separate launches
-    (float4 rows of 64, 128, 192 and a scalar-path grid of 50 x 42 x 37)."""
+@pytest.mark.parametrize("tuning", [None, Tuning(8, 8, 16, 16, 8), Tuning(16, 4, 256, 1, 64), Tuning(4, 16, 32, 8, 8)])
+def test_fused_forces_stencil_equals_separate_launches(ctx, tuning):
+    """Forces + sweep in one launch of one-wave blocks gives the bits of the
+    two separate launches (float4 rows of 64, 128, 192 and a scalar-path
+    grid of 50 x 42 x 37)."""
     ws = _small_ensemble()
     ref = _run_batch(ctx, ws, tuning, 10, fused=0)
-    got = _run_batch(ctx, ws, tuning, 10, fused=fused)
+    got = _run_batch(ctx, ws, tuning, 10, fused=1)
     for (rc, rf), (gc, gf) in zip(ref, got):
         _same(rc, rf, gc, gf)
 
 
-@pytest.mark.parametrize("fused,waves", [(0, 8), (1, 64), (2, 256)])
+@pytest.mark.parametrize("fused,tuning", [(0, Tuning(16, 4, 64, 4, 32)), (1, Tuning(8, 8, 64, 4, 32)),
+                                          (1, Tuning(16, 12, 64, 4, 32)), (1, Tuning(4, 16, 16, 16, 8))])
+def test_rule_inside_compaction_equals_rule_then_compaction(ctx, fused, tuning):
+    """The built-in tumour rule evaluated inside the compaction kernel
+    (survivors written in place, daughters straight to their ranks) gives the
+    bits of the rule kernel followed by compaction, for every tile size."""
+    ws = _small_ensemble()
+    ref = _run_batch(ctx, ws, tuning, 10, fused=0, fused_rule=False)
+    got = _run_batch(ctx, ws, tuning, 10, fused=fused, fused_rule=True)
+    for (rc, rf), (gc, gf) in zip(ref, got):
+        _same(rc, rf, gc, gf)
+
+
+@pytest.mark.parametrize("fused,waves", [(0, 8), (1, 64), (1, 256)])
 def test_capped_force_waves_give_the_same_bits(ctx, fused, waves):
     """Fewer force walkers than 64-cell chunks (each strides through its
     XCD's chunks) gives the bits of one walker per chunk."""
@@ -100,12 +113,14 @@ def test_set_tuning_rejects_bad_shapes(ctx):
         for args in ((10, 4, 64, 4, 32), (16, 5, 64, 4, 32), (16, 4, 64, 3, 32), (16, 4, 64, 4, 0)):
             with pytest.raises(_abi.OncError):
                 b.set_tuning(*args)
-        for f in (-1, 3):
+        for f in (-1, 2):
             with pytest.raises(_abi.OncError):
-                b.set_fused(f)
+                _abi.call("onc_batch_set_fused", b.h, f)
         for wv in (-8, 4, 12):
             with pytest.raises(_abi.OncError):
                 b.set_forces_waves(wv)
+        with pytest.raises(_abi.OncError):
+            _abi.call("onc_batch_set_fused_rule", b.h, 2)
     finally:
         b.close()
 
