@@ -552,13 +552,26 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
             for (int q = 0; q < 8; q++)
                 if (c0 + q < S.n_cols) S.st[par ^ 1u][c0 + q][i] = v[q];
         }
+        // run lengths of equal keys from the wave's run starts: the lanes of
+        // one call hold consecutive cells (lanes 0..last active), and a sim
+        // boundary is itself a run start
+        const onc_u64 act = __ballot(true);
+        const onc_u64 starts = __ballot(kprev != k);
         if (kprev != k) {
             S.bins[(size_t)k * 4u] = i;
             S.bins[(size_t)k * 4u + 2u] = S.step + 1u;
-            // uptake, once per occupied voxel, before the sweep: the bin's
-            // first cell counts its run of equal keys (a few cells)
-            onc_u32 cnt = 1u;
-            while (i + cnt < n && keys[i + cnt] == k) cnt++;
+            // uptake, once per occupied voxel, before the sweep: the run
+            // ends at the next start in the wave, or continues past the
+            // wave's last cell (then counted on from there)
+            const int L = (int)(threadIdx.x & 63u);
+            const onc_u64 above = L == 63 ? 0ull : (starts & ~((2ull << L) - 1ull));
+            onc_u32 cnt;
+            if (above != 0ull) {
+                cnt = (onc_u32)(__builtin_ctzll(above) - L);
+            } else {
+                cnt = (onc_u32)(63 - __builtin_clzll(act) - L + 1);
+                while (i + cnt < n && keys[i + cnt] == k) cnt++;
+            }
             const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
                       vz = onc_vox1(pz, S.dx, S.dims[2]);
             const size_t v = (size_t)vx + (size_t)S.dims[0] * ((size_t)vy + (size_t)S.dims[1] * (size_t)vz);
