@@ -108,6 +108,78 @@ def _sim_cu(sim) -> int:
     return int(v.value)
 
 
+def kernel_rooflines(ctx):
+    """The pure kernels at large sizes, each the median of five calls timed
+    with HIP events around its launches (onc_ctx_timing), against 8 TB/s:
+    plain copy of 256 MiB (the reference rate), stable radix sort of 16M
+    (u32 key, u32 value) pairs at 24 and 27 key bits (3 and 4 passes of
+    16 B/pair; the 256^3 and 512^3 key widths), exclusive scan of 64M u32
+    (8 B/element), one 7-point sweep of a 512^3 binary32 field (8 B/voxel),
+    Morton keys of 16M positions (12 B read + 4 B written per cell)."""
+    import numpy as np
+    from oncolytica_amd.hip import _abi
+    from oncolytica_amd.hip._abi import BC_DIRICHLET, FieldDesc
+    rng = np.random.default_rng(7)
+    out = {}
+    ctx.timing(True)
+
+    def timed(name, alg_bytes, fn, **extra):
+        fn()
+        us = []
+        for _ in range(5):
+            fn()
+            us.append(ctx.last_kernel_us())
+        t = float(np.median(us))
+        gbs = alg_bytes / (t * 1e-6) / 1e9
+        out[name] = {"us": t, "alg_bytes": alg_bytes, "GBps": gbs, "frac": gbs / HBM_PEAK_GBS, **extra}
+        _log(f"kernel {name:18s} {t:9.1f} us {gbs:8.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:5.1f}% of 8 TB/s)")
+
+    nw = 64 << 20
+    a, b = ctx.alloc(4 * nw), ctx.alloc(4 * nw)
+    timed("copy_256MiB", 8.0 * nw, lambda: _abi.call("onc_copy_u32", ctx.h, a.p, b.p, nw), words=nw)
+    a.free()
+    b.free()
+    n = 16 << 20
+    v = ctx.upload(np.arange(n, dtype=np.uint32))
+    ko, vo = ctx.alloc(4 * n), ctx.alloc(4 * n)
+    for bits in (24, 27):
+        k = ctx.upload(rng.integers(0, 1 << bits, n, dtype=np.uint64).astype(np.uint32))
+        P = (bits + 7) // 8
+        timed(f"sort_pairs_16M_{bits}b", 16.0 * P * n,
+              lambda: _abi.call("onc_sort_pairs", ctx.h, k.p, v.p, n, bits, ko.p, vo.p), n=n, key_bits=bits,
+              passes=P)
+        k.free()
+    v.free()
+    ko.free()
+    vo.free()
+    n = 64 << 20
+    x = ctx.upload(rng.integers(0, 16, n, dtype=np.uint64).astype(np.uint32))
+    y = ctx.alloc(4 * n)
+    timed("exscan_64M", 8.0 * n, lambda: _abi.call("onc_exscan_u32", ctx.h, x.p, y.p, n, None), n=n)
+    x.free()
+    y.free()
+    d = 512
+    u = ctx.upload(rng.random(d ** 3, dtype=np.float32))
+    w = ctx.alloc(4 * d ** 3)
+    f = FieldDesc()
+    f.rate, f.bc, f.bc_value, f.lo, f.hi = 0.1, BC_DIRICHLET, 1.0, 0.0, 1.0
+    dims = np.array([d, d, d], dtype=np.int32)
+    timed("stencil7_512", 8.0 * d ** 3,
+          lambda: _abi.call("onc_stencil7", ctx.h, u.p, w.p, dims.ctypes.data_as(_abi._i32p), f), voxels=d ** 3)
+    u.free()
+    w.free()
+    n = 16 << 20
+    px, py, pz = (ctx.upload((rng.random(n) * 512).astype(np.float32)) for _ in range(3))
+    keys = ctx.alloc(4 * n)
+    timed("morton_16M", 16.0 * n,
+          lambda: _abi.call("onc_morton_keys", ctx.h, px.p, py.p, pz.p, n, dims.ctypes.data_as(_abi._i32p), 1.0,
+                            _abi.KEY_MORTON32, keys.p), n=n)
+    for q in (px, py, pz, keys):
+        q.free()
+    ctx.timing(False)
+    return out
+
+
 def ensemble(ctx, n_sims: int, steps: int):
     """Batched uneven ensemble (64 sims, cells log-uniform in [10k, 500k],
     grids from {64, 128, 192}^3, per-sim uptake / division age / hypoxia):
@@ -411,6 +483,8 @@ def main():
                     help="arrival window of the served workload (Poisson, 4 jobs/s; 0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-kernels", action="store_true",
+                    help="skip the pure-kernel rooflines (sort, scan, sweep, Morton at large sizes)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run a plain 256 MiB copy kernel 5 times first (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -577,6 +651,7 @@ def main():
             "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic,
             "alg_bytes_per_launch": dom_bytes, "avg_launch_us": dom_avg}
 
+    kern = None if args.no_kernels else kernel_rooflines(ctx)
     cpu = None if args.no_cpu_baseline else cpu_baseline(w, args.cpu_steps, args.cpu_steps_1t)
     ens = ensemble(ctx, args.ensemble_sims, args.ensemble_steps) if args.ensemble_sims > 0 else None
     served = serve(ctx, args.serve_rate, args.serve_seconds, args.serve_budget_gb) if args.serve_seconds > 0 else None
@@ -604,6 +679,7 @@ def main():
         "roofline": roof,
         "phases": table,
         "phases_separate_launches": table_sep,
+        "kernels": kern,
         "value_separate_launches": sep_value,
         "cpu_baseline": cpu,
         "ensemble": ens,

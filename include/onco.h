@@ -235,6 +235,12 @@ int onc_stencil7(onc_ctx* ctx, const float* in, float* out, const int32_t dims[3
                  const onc_field_desc* f);
 /* plain copy of n 32-bit words (the calibration kernel of HBM byte counters) */
 int onc_copy_u32(onc_ctx* ctx, const uint32_t* src, uint32_t* dst, uint64_t n);
+/* GPU timing of the pure kernels above (no counterpart in the reference,
+ * which has no timing code): with on = 1, each later call records HIP events
+ * on the context stream around its kernel launches only (not its set-up
+ * copies), and onc_ctx_last_kernel_us gives that span in microseconds. */
+int onc_ctx_timing(onc_ctx* ctx, int on);
+int onc_ctx_last_kernel_us(onc_ctx* ctx, double* us);
 /* Philox4x32-10: out[4i..4i+3] = philox(ctr[4i..4i+3], key) */
 int onc_philox(onc_ctx* ctx, const uint32_t* ctr, uint32_t n, uint64_t key, uint32_t* out);
 

@@ -63,8 +63,9 @@ hipError_t onc_launch_morton_keys(const float* px, const float* py, const float*
 hipError_t onc_launch_bins(const onc_u32* keys, onc_u32 n, onc_u32* start, onc_u32* end, hipStream_t s);
 hipError_t onc_launch_philox(const onc_u32* ctr, onc_u32 n, onc_u32 k0, onc_u32 k1, onc_u32* out,
                              hipStream_t s);
+// one block per 4096-element tile (static order, decoupled look-back)
 hipError_t onc_launch_exscan(const onc_u32* in, onc_u32* out, onc_u32 n, onc_u32* total,
-                             onc_u32* status, onc_u32* counter, onc_u32* err, int grid, hipStream_t s);
+                             onc_u32* status, onc_u32* err, hipStream_t s);
 hipError_t onc_launch_fill_u32(onc_u32* p, onc_u32 v, size_t n, hipStream_t s);
 hipError_t onc_launch_field_init(float* u, int X, int Y, int Z, const onc_dfield& f, hipStream_t s);
 hipError_t onc_launch_copy_u32(const onc_u32* src, onc_u32* dst, size_t n, hipStream_t s);

@@ -58,7 +58,24 @@ struct onc_ctx {
     // scratch for pure kernels
     std::vector<void*> scratch;
     size_t scratch_bytes = 0;
+    // optional GPU timing of the pure kernels (onc_ctx_timing)
+    bool timing = false;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    double last_us = 0.0;
 };
+
+// Pure-kernel timing: events on the context stream around the launches of
+// one call (not its set-up copies); read after the call's stream sync.
+static void tmark(onc_ctx* c, int end) {
+    if (c->timing) hipEventRecord(end ? c->t1 : c->t0, c->stream);
+}
+static int tread(onc_ctx* c) {
+    if (!c->timing) return ONC_OK;
+    float ms = 0.0f;
+    CK(hipEventElapsedTime(&ms, c->t0, c->t1));
+    c->last_us = 1000.0 * ms;
+    return ONC_OK;
+}
 
 struct onc_module {
     onc_ctx* ctx = nullptr;
@@ -165,6 +182,8 @@ extern "C" int onc_ctx_destroy(onc_ctx* c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     for (void* p : c->scratch) hipFree(p);
+    if (c->t0) hipEventDestroy(c->t0);
+    if (c->t1) hipEventDestroy(c->t1);
     hipStreamDestroy(c->stream);
     delete c;
     return ONC_OK;
@@ -787,14 +806,13 @@ static onc_u32 force_waves(onc_batch* b) {
     return (onc_u32)((std::max<uint64_t>(n, 8) + 7) / 8 * 8);
 }
 static int grid_forces(onc_batch* b) { return (int)force_waves(b); }
-// Tile kernels: one block per tile the sims' capacity can need (blocks past
-// this step's live tiles leave at once), up to a cap beyond which blocks loop.
-static int grid_tiles(onc_batch* b) {
-    return (int)std::max<uint64_t>(1, std::min<uint64_t>(b->ka.max_tiles, 65535));
-}
-static int grid_ctiles(onc_batch* b) {
-    return (int)std::max<uint64_t>(1, std::min<uint64_t>(b->ka.max_ctiles, 65535));
-}
+// Tile kernels (look-back scans): one block per tile the sims' capacity can
+// need; blocks past this step's live tiles leave at once. A tile waits only
+// on lower tiles, whose blocks were dispatched before it, so every wait ends;
+// a smaller grid whose blocks loop over tiles could hold every resident block
+// on a tile owned by a block that is not yet resident.
+static int grid_tiles(onc_batch* b) { return (int)std::max<uint64_t>(1, b->ka.max_tiles); }
+static int grid_ctiles(onc_batch* b) { return (int)std::max<uint64_t>(1, b->ka.max_ctiles); }
 // Morton: one block per CU at most (each holds an LDS histogram that it adds
 // to global memory once per sim it touches).
 static int grid_morton(onc_batch* b) {
@@ -1137,9 +1155,11 @@ extern "C" int onc_morton_keys(onc_ctx* c, const float* px, const float* py, con
     if (!c || !dims || (n && (!px || !py || !pz || !keys))) return fail(ONC_ERR_INVALID, "NULL argument");
     if (fmt != ONC_KEY_MORTON30 && fmt != ONC_KEY_MORTON32) return fail(ONC_ERR_INVALID, "bad key format");
     CK(hipSetDevice(c->device));
+    tmark(c, 0);
     CK(onc_launch_morton_keys(px, py, pz, n, dims[0], dims[1], dims[2], dx, fmt, keys, c->stream));
+    tmark(c, 1);
     CK(hipStreamSynchronize(c->stream));
-    return ONC_OK;
+    return tread(c);
 }
 
 extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* vin, uint32_t n, int key_bits,
@@ -1203,17 +1223,23 @@ extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* v
     a.hist = (onc_u32*)(B + o_hist);
     a.sort_status = (onc_u32*)(B + o_ss);
     a.comp_status = (onc_u64*)(B + o_cs);
-    const int grid = (int)std::min<size_t>(tiles, (size_t)c->cus * 4);
+    // one block per tile: a tile's look-back waits only on lower tiles, whose
+    // blocks were dispatched before it (a grid of fewer blocks looping over
+    // tiles can hold every resident block on a tile of a block that never
+    // becomes resident)
+    const int grid = (int)tiles;
     CK(onc_launch_plan(a, ONC_PLAN_STEP, st));
-    CK(onc_launch_hist_only(a, items, grid, st));
+    tmark(c, 0);
+    CK(onc_launch_hist_only(a, items, (int)std::min<size_t>(tiles, (size_t)c->cus * 4), st));
     for (int p = 0; p < passes; p++) CK(onc_launch_sort_pass(a, items, p, grid, st));
+    tmark(c, 1);
     CK(hipMemcpyAsync(kout, ds.keys[passes & 1], n * 4ull, hipMemcpyDeviceToDevice, st));
     CK(hipMemcpyAsync(vout, ds.vals[passes & 1], n * 4ull, hipMemcpyDeviceToDevice, st));
     onc_u32 err = 0;
     CK(hipMemcpyAsync(&err, a.err, 4, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     if (err) return fail(ONC_ERR_DEVICE, "sort look-back wait expired");
-    return ONC_OK;
+    return tread(c);
 }
 
 extern "C" int onc_exscan_u32(onc_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total) {
@@ -1229,16 +1255,16 @@ extern "C" int onc_exscan_u32(onc_ctx* c, const uint32_t* in, uint32_t* out, uin
     void* base;
     CKR(ctx_scratch(c, tiles * 8 + 256, &base));
     onc_u64* status = (onc_u64*)base;
-    onc_u32* counter = (onc_u32*)((char*)base + tiles * 8);
-    onc_u32* err = counter + 1;
+    onc_u32* err = (onc_u32*)((char*)base + tiles * 8);
     CK(hipMemsetAsync(base, 0, tiles * 8 + 256, c->stream));
-    const int grid = (int)std::min<size_t>(tiles, (size_t)c->cus * 4);
-    CK(onc_launch_exscan(in, out, n, total, (onc_u32*)status, counter, err, grid, c->stream));
+    tmark(c, 0);
+    CK(onc_launch_exscan(in, out, n, total, (onc_u32*)status, err, c->stream));
+    tmark(c, 1);
     onc_u32 herr = 0;
     CK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
     CK(hipStreamSynchronize(c->stream));
     if (herr) return fail(ONC_ERR_DEVICE, "scan look-back wait expired");
-    return ONC_OK;
+    return tread(c);
 }
 
 extern "C" int onc_bins(onc_ctx* c, const uint32_t* keys, uint32_t n, uint32_t* start, uint32_t* end) {
@@ -1289,9 +1315,11 @@ extern "C" int onc_stencil7(onc_ctx* c, const float* in, float* out, const int32
     a.sbx = sbx;
     a.sby = sby;
     a.sbz = sbz;
+    tmark(c, 0);
     CK(onc_launch_stencil(a, nb, c->stream));
+    tmark(c, 1);
     CK(hipStreamSynchronize(c->stream));
-    return ONC_OK;
+    return tread(c);
 }
 
 extern "C" int onc_philox(onc_ctx* c, const uint32_t* ctr, uint32_t n, uint64_t key, uint32_t* out) {
@@ -1311,10 +1339,31 @@ extern "C" int onc_sim_cell_updates(onc_sim* s, uint64_t* cu) {
     return ONC_OK;
 }
 
+extern "C" int onc_ctx_timing(onc_ctx* c, int on) {
+    if (!c) return fail(ONC_ERR_INVALID, "NULL context");
+    CK(hipSetDevice(c->device));
+    if (on && !c->t0) {
+        CK(hipEventCreate(&c->t0));
+        CK(hipEventCreate(&c->t1));
+    }
+    c->timing = on != 0;
+    c->last_us = 0.0;
+    return ONC_OK;
+}
+
+extern "C" int onc_ctx_last_kernel_us(onc_ctx* c, double* us) {
+    if (!c || !us) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (!c->timing) return fail(ONC_ERR_STATE, "timing is off (onc_ctx_timing)");
+    *us = c->last_us;
+    return ONC_OK;
+}
+
 extern "C" int onc_copy_u32(onc_ctx* c, const uint32_t* src, uint32_t* dst, uint64_t n) {
     if (!c || (n && (!src || !dst))) return fail(ONC_ERR_INVALID, "NULL argument");
     CK(hipSetDevice(c->device));
+    tmark(c, 0);
     CK(onc_launch_copy_u32(src, dst, (size_t)n, c->stream));
+    tmark(c, 1);
     CK(hipStreamSynchronize(c->stream));
-    return ONC_OK;
+    return tread(c);
 }

@@ -128,6 +128,8 @@ _PROTOS = {
     "onc_stencil7": (C.c_int, [_P, _P, _P, _i32p, C.POINTER(FieldDesc)]),
     "onc_philox": (C.c_int, [_P, _P, _u32, C.c_uint64, _P]),
     "onc_copy_u32": (C.c_int, [_P, _P, _P, C.c_uint64]),
+    "onc_ctx_timing": (C.c_int, [_P, C.c_int]),
+    "onc_ctx_last_kernel_us": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -200,6 +202,16 @@ class Context:
 
     def sync(self) -> None:
         call("onc_ctx_sync", self.h)
+
+    def timing(self, on: bool = True) -> None:
+        """GPU timing (HIP events) of the pure-kernel calls that follow."""
+        call("onc_ctx_timing", self.h, 1 if on else 0)
+
+    def last_kernel_us(self) -> float:
+        """GPU time of the last pure-kernel call's launches (timing on)."""
+        v = C.c_double()
+        call("onc_ctx_last_kernel_us", self.h, C.byref(v))
+        return v.value
 
 
 class DeviceBuffer:

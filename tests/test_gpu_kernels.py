@@ -59,7 +59,8 @@ def gpu_sort(ctx, k, v, bits):
 
 
 @pytest.mark.parametrize("n,bits", [(0, 18), (1, 18), (7, 8), (4095, 18), (4096, 18), (4097, 18),
-                                    (100_003, 24), (1_000_000, 24), (1_000_000, 32), (3_000_017, 27)])
+                                    (100_003, 24), (1_000_000, 24), (1_000_000, 32), (3_000_017, 27),
+                                    (16_777_216, 27)])
 def test_sort_pairs(ctx, n, bits):
     print(f"sort n={n} bits={bits}", flush=True)
     rng = np.random.default_rng(n + bits)
@@ -163,3 +164,22 @@ def test_philox_known_answers_and_random(ctx):
     _abi.call("onc_philox", ctx.h, c.p, len(ctr), 0x123456789ABCDEF, o.p)
     np.testing.assert_array_equal(o.download(np.uint32, ctr.size).reshape(-1, 4),
                                   O.philox(ctr, 0x123456789ABCDEF))
+
+
+def test_pure_kernel_timing(ctx):
+    """onc_ctx_timing: a timed call reports a positive GPU span, results are
+    unchanged, and reading the span with timing off is an error."""
+    dims = (64, 64, 64)
+    u = np.random.default_rng(2).random(int(np.prod(dims))).astype(np.float32)
+    f = _fd(rate=0.1, bc_value=1.0)
+    bi, bo = ctx.upload(u), ctx.alloc(u.nbytes)
+    ctx.timing(True)
+    try:
+        _abi.call("onc_stencil7", ctx.h, bi.p, bo.p, _dims(dims).ctypes.data_as(_abi._i32p), f)
+        assert ctx.last_kernel_us() > 0.0
+        np.testing.assert_array_equal(bo.download(np.float32, u.size).view(np.uint32),
+                                      O.stencil7(u, dims, f, "f32").view(np.uint32))
+    finally:
+        ctx.timing(False)
+    with pytest.raises(_abi.OncError):
+        ctx.last_kernel_us()
