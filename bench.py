@@ -383,12 +383,12 @@ def stress(ctx, steps: int = 300, burst=(100, 110), checkpoints=(0, 105)):
 # rocprofv3 runs whose outputs `--summarize DIR` reduces (each under its own
 # subdirectory of DIR, written with `-d DIR/<sub> -o run`)
 PROFILE_RUNS = {
-    "kt": "rocprofv3 --kernel-trace --stats -d {d}/kt -o run -- python3 bench.py --steps 100 --warmup 5 "
+    "kt": "rocprofv3 --kernel-trace --stats -d {d}/kt -o run -- python3 bench.py "
           "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0",
-    "FETCH_SIZE": "rocprofv3 --pmc FETCH_SIZE -d {d}/FETCH_SIZE -o run -- python3 bench.py --steps 20 --warmup 2 "
-                  "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0 --calibrate",
-    "WRITE_SIZE": "rocprofv3 --pmc WRITE_SIZE -d {d}/WRITE_SIZE -o run -- python3 bench.py --steps 20 --warmup 2 "
-                  "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0 --calibrate",
+    "FETCH_SIZE": "rocprofv3 --pmc FETCH_SIZE -d {d}/FETCH_SIZE -o run -- python3 bench.py --steps 100 "
+                  "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0 --no-kernels --calibrate",
+    "WRITE_SIZE": "rocprofv3 --pmc WRITE_SIZE -d {d}/WRITE_SIZE -o run -- python3 bench.py --steps 100 "
+                  "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0 --no-kernels --calibrate",
 }
 KERNEL_PHASE = {"k_plan": "plan", "k_morton": "morton", "k_sort_pass": "sort", "k_permute_bins": "permute_bins",
                 "k_forces": "forces", "k_stencil": "stencil", "k_rule_tumor": "rule", "k_compact": "compact",
@@ -410,7 +410,7 @@ def _rocpd(path: str):
     return sqlite3.connect(dbs[-1])
 
 
-def summarize(d: str, workload: str = "large_1m_256"):
+def summarize(d: str, workload: str = "large_1m_256", outdir: str = None):
     """Reduce the PROFILE_RUNS outputs under `d` to profiles/: the kernel-trace
     statistics, and HBM traffic per launch from FETCH_SIZE / WRITE_SIZE (KiB per
     dispatch) scaled by what the same counters read for the plain copy kernel
@@ -430,8 +430,9 @@ def summarize(d: str, workload: str = "large_1m_256"):
     cal = {c: COPY_BYTES / 1024.0 / pmc["copy"][c] for c in ("FETCH_SIZE", "WRITE_SIZE")}
     traffic = {ph: 1024.0 * (v["FETCH_SIZE"] * cal["FETCH_SIZE"] + v["WRITE_SIZE"] * cal["WRITE_SIZE"])
                for ph, v in pmc.items() if ph != "copy" and len(v) == 2}
-    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
-    tj = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    outdir = outdir or os.path.join(ROOT, "profiles")
+    os.makedirs(outdir, exist_ok=True)
+    tj = os.path.join(outdir, f"traffic_{workload}.json")
     with open(tj, "w") as f:
         json.dump({**{k: round(v) for k, v in traffic.items()},
                    "_calibration": {"copy_bytes_each_way": COPY_BYTES,
@@ -456,7 +457,7 @@ def summarize(d: str, workload: str = "large_1m_256"):
     for ph, v in sorted(pmc.items()):
         if ph in traffic:
             lines.append(f"| {ph} | {v['FETCH_SIZE']:.0f} | {v['WRITE_SIZE']:.0f} | {traffic[ph]:,.0f} |")
-    md = os.path.join(ROOT, "profiles", f"kernels_{workload}.md")
+    md = os.path.join(outdir, f"kernels_{workload}.md")
     with open(md, "w") as f:
         f.write("\n".join(lines) + "\n")
     print(f"wrote {tj} and {md}")
@@ -464,7 +465,7 @@ def summarize(d: str, workload: str = "large_1m_256"):
 
 def main():
     if len(sys.argv) >= 3 and sys.argv[1] == "--summarize":
-        summarize(sys.argv[2])
+        summarize(sys.argv[2], outdir=sys.argv[3] if len(sys.argv) > 3 else None)
         return
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
