@@ -536,7 +536,6 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
         onc_gcol const* from = fromd ? S.dau2 : S.st[par];
         const onc_u32 k = keys[i];
         const onc_u32 kprev = i > 0u ? keys[i - 1u] : ~k;
-        const onc_u32 knext = i + 1u < n ? keys[i + 1u] : ~k;
         // gather in chunks of eight columns: loads first, then stores
         float px = 0.0f, py = 0.0f, pz = 0.0f;
         for (int c0 = 0; c0 < S.n_cols; c0 += 8) {
@@ -559,11 +558,8 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
         const onc_u64 act = __ballot(true);
         const onc_u64 starts = __ballot(kprev != k);
         if (kprev != k) {
-            S.bins[(size_t)k * 4u] = i;
-            S.bins[(size_t)k * 4u + 2u] = S.step + 1u;
-            // uptake, once per occupied voxel, before the sweep: the run
-            // ends at the next start in the wave, or continues past the
-            // wave's last cell (then counted on from there)
+            // the run ends at the next start in the wave, or continues past
+            // the wave's last cell (then counted on from there)
             const int L = (int)(threadIdx.x & 63u);
             const onc_u64 above = L == 63 ? 0ull : (starts & ~((2ull << L) - 1ull));
             onc_u32 cnt;
@@ -573,6 +569,9 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
                 cnt = (onc_u32)(63 - __builtin_clzll(act) - L + 1);
                 while (i + cnt < n && keys[i + cnt] == k) cnt++;
             }
+            // the whole bin entry {start, end, epoch, 0} in one 16-byte store
+            ((uint4*)S.bins)[k] = make_uint4(i, i + cnt, S.step + 1u, 0u);
+            // uptake, once per occupied voxel, before the sweep
             const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
                       vz = onc_vox1(pz, S.dx, S.dims[2]);
             const size_t v = (size_t)vx + (size_t)S.dims[0] * ((size_t)vy + (size_t)S.dims[1] * (size_t)vz);
@@ -584,7 +583,6 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
                 u[v] = w < fd.lo ? fd.lo : w;
             }
         }
-        if (knext != k) S.bins[(size_t)k * 4u + 1u] = i + 1u;
     });
 }
 
