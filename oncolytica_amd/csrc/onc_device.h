@@ -279,6 +279,25 @@ __device__ __forceinline__ int onc_find_seg(const onc_u32* prefix, onc_u32 nsegs
 // it (and what is loaded through it) in scalar registers.
 __device__ __forceinline__ onc_u32 onc_uniform(onc_u32 x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Largest s in [0, n) with prefix[s] <= g (prefix non-decreasing, prefix[0]
+// = 0), for g equal on every lane; all 64 lanes of the wave must be active.
+// Lane l loads prefix[base + l] and a ballot counts the entries <= g: one
+// memory round trip per 64 entries instead of a binary search of dependent
+// loads (a 64-sim batch: one load instead of six).
+__device__ __forceinline__ int onc_find_seg_wave(const onc_u32* prefix, onc_u32 n, onc_u32 g) {
+    const onc_u32 lane = threadIdx.x & 63u;
+    int s = 0;
+    for (onc_u32 base = 0; base < n; base += 64u) {
+        const onc_u32 idx = base + lane;
+        const bool le = idx < n && prefix[idx] <= g;
+        const int c = __popcll(__ballot(le));
+        if (c == 0) break;
+        s = (int)base + c - 1;
+        if (c < 64) break;
+    }
+    return (int)onc_uniform((onc_u32)s);
+}
+
 // XCD-aware block order. The 8 XCDs of an MI355X (each with its own L2) take
 // workgroups round-robin by blockIdx; this bijection gives XCD k a contiguous
 // run of logical block ids, so blocks that share halo data or neighbour bins
@@ -322,7 +341,7 @@ __device__ __forceinline__ void onc_for_cells_wave(const onc_u32* cell_prefix, o
     for (onc_u32 c = c0; c < c1; c++) {
         const onc_u32 base = c * 64u;
         const onc_u32 g = base + lane;
-        const int s0 = (int)onc_uniform((onc_u32)onc_find_seg(cell_prefix, nsims, base));
+        const int s0 = onc_find_seg_wave(cell_prefix, nsims, base);
         const onc_u32 lo = cell_prefix[s0], hi = cell_prefix[s0 + 1];
         if (base + 63u < hi) {
             f(s0, g - lo);
@@ -356,7 +375,7 @@ __device__ __forceinline__ void onc_for_chunks_xcd(const onc_u32* cell_prefix, o
         if (c >= nchunks) break;
         const onc_u32 base = c * 64u;
         const onc_u32 g = base + lane;
-        const int s0 = (int)onc_uniform((onc_u32)onc_find_seg(cell_prefix, nsims, base));
+        const int s0 = onc_find_seg_wave(cell_prefix, nsims, base);
         const onc_u32 lo = cell_prefix[s0], hi = cell_prefix[s0 + 1];
         if (base + 63u < hi) {
             f(s0, g - lo);

@@ -272,8 +272,8 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
         const onc_u32 ce = min(c + (onc_u32)ONC_MORTON_CHUNKS, c1);
         const onc_u32 base = c << 8;
         const onc_u32 last = min(ce << 8, total) - 1u;
-        const int sa = (int)onc_uniform((onc_u32)onc_find_seg(a.cell_prefix, a.nsims, base));
-        const int sb = (int)onc_uniform((onc_u32)onc_find_seg(a.cell_prefix, a.nsims, last));
+        const int sa = onc_find_seg_wave(a.cell_prefix, a.nsims, base);
+        const int sb = onc_find_seg_wave(a.cell_prefix, a.nsims, last);
         if (sa == sb) {
             // up to eight chunks of one sim: all position loads in flight at once
             if (sa != cur) {
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
             // chunks across a sim boundary: one chunk at a time
             for (onc_u32 cc = c; cc < ce; cc++) {
                 const onc_u32 b0 = cc << 8;
-                const int s0 = (int)onc_uniform((onc_u32)onc_find_seg(a.cell_prefix, a.nsims, b0));
+                const int s0 = onc_find_seg_wave(a.cell_prefix, a.nsims, b0);
                 if (s0 != cur) {
                     if (cur >= 0) flush(cur);
                     cur = s0;
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(256) void k_hist_only(onc_kargs a) {
     const onc_u32 total = a.tile_prefix[a.nsims];
     const onc_u32 P = a.passes;
     for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
-        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
+        const int s = onc_find_seg_wave(a.tile_prefix, a.nsims, g);
         const onc_dsim& S = a.sims[s];
         const onc_u32 n = S.n_live;
         const onc_u32 base = (g - a.tile_prefix[s]) * (onc_u32)(256 * ITEMS);
@@ -408,7 +408,7 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
     for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
         for (int i = tid; i < 4 * 256; i += 256) s_wc[i] = 0u;
         __syncthreads();
-        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.tile_prefix, a.nsims, g));
+        const int s = onc_find_seg_wave(a.tile_prefix, a.nsims, g);
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.tile_prefix[s];
         const onc_u32 n = S.n_live;
@@ -902,7 +902,7 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
 // bx = min(sbx, NT), by = NT / bx.
 template <int NT>
 __device__ __forceinline__ void onc_stencil_block(const onc_kargs& a, onc_u32 b) {
-    const int jb = (int)onc_uniform((onc_u32)onc_find_seg(a.jobs_prefix, a.njobs, b));
+    const int jb = onc_find_seg_wave(a.jobs_prefix, a.njobs, b);
     const onc_u32 job = onc_uniform(a.jobs[jb]);
     const onc_dsim& S = a.sims[job >> 8];
     if (S.frozen) return;
@@ -1189,7 +1189,7 @@ __global__ __launch_bounds__(256) void k_compact(onc_kargs a) {
     const int tid = threadIdx.x;
     const onc_u32 total = a.ctile_prefix[a.nsims];
     for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {  // static tile order, as in k_sort_pass
-        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.ctile_prefix, a.nsims, g));
+        const int s = onc_find_seg_wave(a.ctile_prefix, a.nsims, g);
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.ctile_prefix[s];
         const onc_u32 ntiles_s = a.ctile_prefix[s + 1] - a.ctile_prefix[s];
@@ -1248,7 +1248,7 @@ __global__ __launch_bounds__(256) void k_rule_compact(onc_kargs a) {
     const int tid = threadIdx.x;
     const onc_u32 total = a.ctile_prefix[a.nsims];
     for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {  // static tile order, as in k_sort_pass
-        const int s = (int)onc_uniform((onc_u32)onc_find_seg(a.ctile_prefix, a.nsims, g));
+        const int s = onc_find_seg_wave(a.ctile_prefix, a.nsims, g);
         const onc_dsim& S = a.sims[s];
         const onc_u32 t = g - a.ctile_prefix[s];
         const onc_u32 ntiles_s = a.ctile_prefix[s + 1] - a.ctile_prefix[s];
