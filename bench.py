@@ -242,7 +242,7 @@ def ensemble(ctx, n_sims: int, steps: int):
             "speedup": (bcu / bt) / (plain_cu / plain_t), "cell_updates": bcu,
             "bit_identical_to_plain": bool(same), "sims_compared": len(plain_out),
             "tuning": {"sort_items": tun.items, "compact_items": tun.citems, "sbx": tun.sbx, "sby": tun.sby,
-                       "sbz": tun.sbz},
+                       "sbz": tun.sbz, "fused": tun.fused},
             "autotune_s": t_tune}
 
 
