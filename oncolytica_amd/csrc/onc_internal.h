@@ -12,6 +12,16 @@
 
 #define ONC_BLOCK 256
 
+// Sticky device error bits (onc_kargs::err): a bounded look-back wait expired
+// in a sort pass (1), the compaction scan (2) or the standalone scan (4); a
+// voxel held more than ONC_MAX_BIN_CELLS cells (8; the force walk keeps bin
+// lengths in 16 bits).
+#define ONC_DERR_SORT 1u
+#define ONC_DERR_COMPACT 2u
+#define ONC_DERR_SCAN 4u
+#define ONC_DERR_BIG_BIN 8u
+#define ONC_MAX_BIN_CELLS 65535u
+
 // Width in float4 columns of a float4 stencil tile: the smallest power of two
 // >= X/4, capped at sbx (a power of two <= 256). Lanes of one row segment of
 // min(width, 64) lanes hold consecutive float4 of one row, so x halos pass

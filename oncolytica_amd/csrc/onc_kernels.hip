@@ -571,6 +571,7 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
             }
             // the whole bin entry {start, end, epoch, 0} in one 16-byte store
             ((uint4*)S.bins)[k] = make_uint4(i, i + cnt, S.step + 1u, 0u);
+            if (cnt > ONC_MAX_BIN_CELLS) atomicOr(a.err, ONC_DERR_BIG_BIN);
             // uptake, once per occupied voxel, before the sweep
             const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
                       vz = onc_vox1(pz, S.dx, S.dims[2]);
@@ -607,7 +608,7 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 template <class F, class G>
 __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
                                             onc_u32 self, const float4* __restrict__ pos4, onc_u32 (*s_rs)[ONC_FORCES_BLOCK],
-                                            onc_u32 (*s_re)[ONC_FORCES_BLOCK], F pair, G batch_end) {
+                                            unsigned short (*s_rl)[ONC_FORCES_BLOCK], F pair, G batch_end) {
     const int tid = threadIdx.x & (ONC_FORCES_BLOCK - 1);  // lane: LDS slots are per wave
     const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
               vz = onc_vox1(pz, S.dx, S.dims[2]);
@@ -628,14 +629,15 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, fl
         for (int q = 0; q < 9; q++) {
             if (b[q].z == epoch && b[q].y > b[q].x) {
                 s_rs[nr][tid] = b[q].x;
-                s_re[nr][tid] = b[q].y;
+                // clamped: a longer bin (reported as an error) still ends the walk
+                s_rl[nr][tid] = (unsigned short)min(b[q].y - b[q].x, ONC_MAX_BIN_CELLS);
                 nr++;
             }
         }
         onc_u32 k = 0, j = 0, e = 0;
         if (nr > 0) {
             j = s_rs[0][tid];
-            e = s_re[0][tid];
+            e = j + s_rl[0][tid];
         }
         while (k < nr) {
             onc_u32 jj[8];
@@ -648,7 +650,7 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, fl
                         k++;
                         if (k < nr) {
                             j = s_rs[k][tid];
-                            e = s_re[k][tid];
+                            e = j + s_rl[k][tid];
                         }
                     }
                 }
@@ -667,9 +669,14 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, fl
 // LDS of one forces wave: per-thread list of interacting neighbours
 // [slot][thread] (bank-conflict free) and the non-empty bin ranges of the
 // current z-plane.
+// Range lengths are 16-bit (a bin of more than 65535 cells is reported by the
+// permute kernel as a device error): 7.5 KB per wave lets LDS hold as many
+// force waves as their registers allow (8.7 KB with 32-bit ends held the
+// kernel at 18 waves per CU instead of 20; forces 3.5 % faster).
 struct onc_forces_lds {
     onc_u32 pairs[ONC_PAIR_LIST][ONC_FORCES_BLOCK];
-    onc_u32 rs[9][ONC_FORCES_BLOCK], re[9][ONC_FORCES_BLOCK];
+    onc_u32 rs[9][ONC_FORCES_BLOCK];
+    unsigned short rl[9][ONC_FORCES_BLOCK];  // range lengths
 };
 
 // Forces of cell i of sim s through global memory (bins table, packed
@@ -678,7 +685,7 @@ struct onc_forces_lds {
 __device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u32 i, onc_forces_lds& L) {
     onc_u32 (*s_pairs)[ONC_FORCES_BLOCK] = L.pairs;
     onc_u32 (*s_rs)[ONC_FORCES_BLOCK] = L.rs;
-    onc_u32 (*s_re)[ONC_FORCES_BLOCK] = L.re;
+    unsigned short (*s_rl)[ONC_FORCES_BLOCK] = L.rl;
     const int tid = threadIdx.x & (ONC_FORCES_BLOCK - 1);  // lane: LDS slots are per wave
     const onc_dsim& S = a.sims[s];
     const onc_u32 par = S.par;
@@ -731,7 +738,7 @@ __device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u
     // once any lane of the wave holds more, every lane still walking
     // flushes together. A flush taken by one lane at a time would run
     // the wave through the pair law once per lane.
-    forces_walk(S, epoch, px, py, pz, i, p4, s_rs, s_re,
+    forces_walk(S, epoch, px, py, pz, i, p4, s_rs, s_rl,
                 [&](const float4& o, onc_u32 j) {
                     const float ddx = px - o.x;
                     const float ddy = py - o.y;

@@ -980,7 +980,14 @@ static int batch_sync_impl(onc_batch* b) {
     if (b->dirty) return ONC_OK;
     onc_u32 err = 0;
     CK(hipMemcpy(&err, b->ka.err, 4, hipMemcpyDeviceToHost));
-    if (err) return fail(ONC_ERR_DEVICE, "a kernel's bounded look-back wait expired (code " + std::to_string(err) + ")");
+    if (err) {
+        std::string m;
+        if (err & (ONC_DERR_SORT | ONC_DERR_COMPACT | ONC_DERR_SCAN))
+            m += "a kernel's bounded look-back wait expired (code " + std::to_string(err & 7u) + "); ";
+        if (err & ONC_DERR_BIG_BIN)
+            m += "a voxel held more than " + std::to_string(ONC_MAX_BIN_CELLS) + " cells (the force walk's limit); ";
+        return fail(ONC_ERR_DEVICE, m);
+    }
     return ONC_OK;
 }
 

@@ -137,3 +137,25 @@ def test_batch_equals_alone(ctx):
     b.close()
     for s in sims:
         s.close()
+
+
+def test_voxel_over_force_walk_limit_is_reported(ctx):
+    """More than 65535 cells in one voxel (here 66,000 at one point) exceed
+    the force walk's bin length: the step's sync reports a device error
+    instead of returning forces computed from a clipped bin."""
+    from oncolytica_amd.hip import _abi
+    w = W.plain_c1()
+    d = w.desc()
+    d.capacity = 140_000
+    n = 66_000
+    cols = W.initial_cells(w, n)
+    for c in range(3):
+        cols[c][:] = np.float32(32.5)
+    sim = Sim(ctx, d)
+    try:
+        sim.upload_cells(cols)
+        sim.step(1)
+        with pytest.raises(_abi.OncError, match="65535"):
+            sim.sync()
+    finally:
+        sim.close()
