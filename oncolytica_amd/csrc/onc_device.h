@@ -132,6 +132,18 @@ struct onc_kargs {
 };
 
 // ---- small helpers ---------------------------------------------------------
+// Streaming (non-temporal) 16-byte store and load: the line is not kept for
+// reuse in the caches (global_store / global_load with the nt bit).
+typedef float onc_f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void onc_st_nt(float4* p, float4 v) {
+    const onc_f4v t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, (onc_f4v*)p);
+}
+__device__ __forceinline__ float4 onc_ld_nt(const float4* p) {
+    const onc_f4v t = __builtin_nontemporal_load((const onc_f4v*)p);
+    return make_float4(t.x, t.y, t.z, t.w);
+}
+
 __device__ __forceinline__ float onc_clampf(float x, float lo, float hi) {
     return x < lo ? lo : (x > hi ? hi : x);
 }

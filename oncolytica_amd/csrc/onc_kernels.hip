@@ -346,36 +346,51 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
     for (onc_u32 i = gid; i < tc; i += nthreads) a.comp_status[i] = 0ull;
 }
 
-// Histograms only (keys already in keys[0]); used by onc_sort_pairs.
+// Histograms only (keys already in keys[0]); used by onc_sort_pairs. A block
+// accumulates the counts of all its tiles of one sim in LDS and adds them to
+// the device histograms once, into copy blockIdx % ONC_HIST_COPIES: flushing
+// per tile would put every tile's 256 x passes atomics on the same few
+// hundred words, which serialise at their memory channel.
 template <int ITEMS>
 __global__ __launch_bounds__(256) void k_hist_only(onc_kargs a) {
     __shared__ onc_u32 s_hist[ONC_DEV_MAX_PASSES * 256];
     const int tid = threadIdx.x;
     const onc_u32 total = a.tile_prefix[a.nsims];
     const onc_u32 P = a.passes;
+    const onc_u32 copy = blockIdx.x % ONC_HIST_COPIES;
+    for (onc_u32 i = tid; i < P * 256u; i += 256) s_hist[i] = 0u;
+    __syncthreads();
+    int cur = -1;
+    auto flush = [&](int s) {
+        __syncthreads();
+        for (onc_u32 p = 0; p < P; p++) {
+            const onc_u32 v = s_hist[p * 256u + tid];
+            if (v) atomicAdd(&a.hist[(((size_t)copy * a.nsims + (size_t)s) * 4u + p) * 256u + tid], v);
+            s_hist[p * 256u + tid] = 0u;
+        }
+        __syncthreads();
+    };
     for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
         const int s = onc_find_seg_wave(a.tile_prefix, a.nsims, g);
+        if (s != cur) {
+            if (cur >= 0) flush(cur);
+            cur = s;
+        }
         const onc_dsim& S = a.sims[s];
         const onc_u32 n = S.n_live;
         const onc_u32 base = (g - a.tile_prefix[s]) * (onc_u32)(256 * ITEMS);
-        for (onc_u32 i = tid; i < P * 256u; i += 256) s_hist[i] = 0u;
-        __syncthreads();
+        const onc_u32* __restrict__ kin = S.keys[0];
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
-            onc_u32 i = base + (onc_u32)(k * 256 + tid);
+            const onc_u32 i = base + (onc_u32)(k * 256 + tid);
             if (i < n) {
-                onc_u32 key = S.keys[0][i];
+                const onc_u32 key = __builtin_nontemporal_load(kin + i);
                 for (onc_u32 p = 0; p < P; p++) atomicAdd(&s_hist[p * 256u + ((key >> (8u * p)) & 255u)], 1u);
             }
         }
-        __syncthreads();
-        for (onc_u32 p = 0; p < P; p++) {
-            onc_u32 v = s_hist[p * 256u + tid];
-            if (v) atomicAdd(&a.hist[((size_t)s * 4u + p) * 256u + tid], v);
-            a.sort_status[((size_t)p * a.max_tiles + g) * 256u + tid] = 0u;
-        }
-        __syncthreads();
+        for (onc_u32 p = 0; p < P; p++) a.sort_status[((size_t)p * a.max_tiles + g) * 256u + tid] = 0u;
     }
+    if (cur >= 0) flush(cur);
 }
 
 // ---------------------------------------------------------------------------
@@ -419,6 +434,7 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
         onc_u32* kout = S.keys[(pass + 1) & 1];
         onc_u32* vout = S.vals[(pass + 1) & 1];
 
+        // keys and values are read once: streaming loads
         onc_u32 key[ITEMS], val[ITEMS], rnk[ITEMS];
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
@@ -426,8 +442,8 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
             key[k] = 0u;
             val[k] = 0u;
             if (idx < cnt) {
-                key[k] = kin[base + idx];
-                val[k] = vin[base + idx];
+                key[k] = __builtin_nontemporal_load(kin + base + idx);
+                val[k] = __builtin_nontemporal_load(vin + base + idx);
             }
         }
         // rank inside the wave, items in (k, lane) order = input order
@@ -894,7 +910,9 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
             if (onc_is_fixed(fd, S.dims, x0 + 2, y, 0)) o.z = fd.bc_value;
             if (onc_is_fixed(fd, S.dims, x0 + 3, y, 0)) o.w = fd.bc_value;
         }
-        *(float4*)(out + v) = o;
+        // the next sweep reads this plane from HBM anyway: a streaming
+        // store keeps it from evicting the planes this sweep still reads
+        onc_st_nt((float4*)(out + v), o);
         cm = c;
         c = cp;
         ym = nym;
@@ -1353,38 +1371,74 @@ __global__ __launch_bounds__(256) void k_fill_u32(onc_u32* p, onc_u32 v, size_t 
 }
 
 // Standalone exclusive scan: single pass, decoupled look-back, 64-bit status
-// words (2 flag bits, 32-bit running sum).
-template <int ITEMS>
+// words (2 flag bits, 32-bit running sum). A tile is 256 x ITEMS elements
+// held in registers: wave w owns the contiguous run [w, w+1) x 64 ITEMS of
+// it, and lane l holds the 16-byte groups at k * 256 + 4 l (k < ITEMS / 4),
+// so every load and store instruction of a wave covers 1 KB in one piece
+// and no LDS transpose is needed. Per group the wave scans the lane sums
+// with shuffles; four wave totals and one look-back give the tile's
+// offsets. Large tiles (16384 elements) keep the look-backs few: the scan
+// is bound by the latency of the look-back chain, not by the bytes moved,
+// once tiles are small. Streaming loads and stores: every element is read
+// and written once. V4 = false reads and writes the same groups one element
+// at a time (pointers that are not 16-byte aligned).
+template <int ITEMS, bool V4>
 __global__ __launch_bounds__(256) void k_exscan(const onc_u32* __restrict__ in, onc_u32* __restrict__ out,
                                                 onc_u32 n, onc_u32* total, onc_u64* status, onc_u32* err) {
     // one block per tile, in static order: a tile's look-back waits only on
     // lower tiles, whose blocks were dispatched before it (a shared tile
     // counter would serialise every block on one device atomic)
+    constexpr int K = ITEMS / 4;
     constexpr int TILE = 256 * ITEMS;
-    __shared__ onc_u32 s_v[TILE];
-    __shared__ onc_u32 s_tmp[8];
+    __shared__ onc_u32 s_w[4];
     __shared__ onc_u32 s_pre;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const onc_u32 ntiles = (n + TILE - 1) / TILE;
     const onc_u32 g = blockIdx.x;
     if (g >= ntiles) return;
-    const onc_u32 base = g * (onc_u32)TILE;
-    const onc_u32 cnt = min((onc_u32)TILE, n - base);
+    const onc_u32 lim = n - g * (onc_u32)TILE;  // elements from the tile start to the end
+    const onc_u32 wbase = (onc_u32)w * (onc_u32)(64 * ITEMS) + (onc_u32)lane * 4u;
+    const onc_u32* __restrict__ src = in + (size_t)g * TILE;
+    onc_u32* __restrict__ dst = out + (size_t)g * TILE;
+    uint4 x[K];
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        onc_u32 j = (onc_u32)(k * 256 + tid);
-        s_v[j] = j < cnt ? in[base + j] : 0u;
+    for (int k = 0; k < K; k++) {
+        const onc_u32 e = wbase + (onc_u32)(k * 256);
+        if (V4 && e + 4u <= lim) {
+            typedef onc_u32 v4u __attribute__((ext_vector_type(4)));
+            const v4u t = __builtin_nontemporal_load((const v4u*)(src + e));
+            x[k] = make_uint4(t.x, t.y, t.z, t.w);
+        } else {
+            x[k].x = e < lim ? __builtin_nontemporal_load(src + e) : 0u;
+            x[k].y = e + 1u < lim ? __builtin_nontemporal_load(src + e + 1u) : 0u;
+            x[k].z = e + 2u < lim ? __builtin_nontemporal_load(src + e + 2u) : 0u;
+            x[k].w = e + 3u < lim ? __builtin_nontemporal_load(src + e + 3u) : 0u;
+        }
     }
+    // exclusive prefix of each group inside the wave's run
+    onc_u32 gex[K];
+    onc_u32 wtot = 0u;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const onc_u32 v = x[k].x + x[k].y + x[k].z + x[k].w;
+        onc_u32 y = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const onc_u32 t = __shfl_up(y, o, 64);
+            if (lane >= o) y += t;
+        }
+        gex[k] = wtot + y - v;
+        wtot += __shfl(y, 63, 64);
+    }
+    if (lane == 0) s_w[w] = wtot;
     __syncthreads();
-    onc_u32 v[ITEMS];
-    onc_u32 sum = 0u;
+    onc_u32 before = 0u, tot = 0u;
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        v[k] = s_v[tid * ITEMS + k];
-        sum += v[k];
+    for (int i = 0; i < 4; i++) {
+        const onc_u32 t = s_w[i];
+        if (i < w) before += t;
+        tot += t;
     }
-    onc_u32 tot;
-    const onc_u32 ex = block_exscan256(sum, s_tmp, &tot);
     if (tid < 64) {
         onc_u32 pre = 0u, unused = 0u;
         onc_u64* mine = &status[g];
@@ -1401,17 +1455,28 @@ __global__ __launch_bounds__(256) void k_exscan(const onc_u32* __restrict__ in, 
         }
     }
     __syncthreads();
-    onc_u32 run = s_pre + ex;
+    const onc_u32 off = s_pre + before;
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        s_v[tid * ITEMS + k] = run;
-        run += v[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        onc_u32 j = (onc_u32)(k * 256 + tid);
-        if (j < cnt) out[base + j] = s_v[j];
+    for (int k = 0; k < K; k++) {
+        const onc_u32 e = wbase + (onc_u32)(k * 256);
+        onc_u32 r = off + gex[k];
+        const onc_u32 o0 = r;
+        r += x[k].x;
+        const onc_u32 o1 = r;
+        r += x[k].y;
+        const onc_u32 o2 = r;
+        r += x[k].z;
+        const onc_u32 o3 = r;
+        if (V4 && e + 4u <= lim) {
+            typedef onc_u32 v4u __attribute__((ext_vector_type(4)));
+            const v4u t = {o0, o1, o2, o3};
+            __builtin_nontemporal_store(t, (v4u*)(dst + e));
+        } else {
+            if (e < lim) __builtin_nontemporal_store(o0, dst + e);
+            if (e + 1u < lim) __builtin_nontemporal_store(o1, dst + e + 1u);
+            if (e + 2u < lim) __builtin_nontemporal_store(o2, dst + e + 2u);
+            if (e + 3u < lim) __builtin_nontemporal_store(o3, dst + e + 3u);
+        }
     }
 }
 
@@ -1504,8 +1569,13 @@ hipError_t onc_launch_philox(const onc_u32* ctr, onc_u32 n, onc_u32 k0, onc_u32 
 hipError_t onc_launch_exscan(const onc_u32* in, onc_u32* out, onc_u32 n, onc_u32* total, onc_u32* status,
                              onc_u32* err, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    const onc_u32 tiles = (n + 256u * 16u - 1u) / (256u * 16u);
-    hipLaunchKernelGGL(k_exscan<16>, dim3(tiles), dim3(256), 0, s, in, out, n, total, (onc_u64*)status, err);
+    const onc_u32 tiles = (n + ONC_SCAN_TILE - 1u) / ONC_SCAN_TILE;
+    if ((((uintptr_t)in) | ((uintptr_t)out)) & 15u)
+        hipLaunchKernelGGL((k_exscan<ONC_SCAN_TILE / 256, false>), dim3(tiles), dim3(256), 0, s, in, out, n, total,
+                           (onc_u64*)status, err);
+    else
+        hipLaunchKernelGGL((k_exscan<ONC_SCAN_TILE / 256, true>), dim3(tiles), dim3(256), 0, s, in, out, n, total,
+                           (onc_u64*)status, err);
     return hipGetLastError();
 }
 hipError_t onc_launch_fill_u32(onc_u32* p, onc_u32 v, size_t n, hipStream_t s) {

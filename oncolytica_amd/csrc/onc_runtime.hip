@@ -1260,8 +1260,7 @@ extern "C" int onc_exscan_u32(onc_ctx* c, const uint32_t* in, uint32_t* out, uin
         CK(hipStreamSynchronize(c->stream));
         return ONC_OK;
     }
-    const size_t tile = 256 * 16;
-    const size_t tiles = (n + tile - 1) / tile;
+    const size_t tiles = (n + ONC_SCAN_TILE - 1) / ONC_SCAN_TILE;
     void* base;
     CKR(ctx_scratch(c, tiles * 8 + 256, &base));
     onc_u64* status = (onc_u64*)base;

@@ -59,8 +59,8 @@ def gpu_sort(ctx, k, v, bits):
 
 
 @pytest.mark.parametrize("n,bits", [(0, 18), (1, 18), (7, 8), (4095, 18), (4096, 18), (4097, 18),
-                                    (100_003, 24), (1_000_000, 24), (1_000_000, 32), (3_000_017, 27),
-                                    (16_777_216, 27)])
+                                    (100_003, 24), (1_000_000, 24), (1_000_000, 32), (2_105_345, 32),
+                                    (3_000_017, 27), (16_777_216, 27)])
 def test_sort_pairs(ctx, n, bits):
     print(f"sort n={n} bits={bits}", flush=True)
     rng = np.random.default_rng(n + bits)
@@ -86,7 +86,7 @@ def test_sort_morton_like_keys_many_duplicates(ctx):
     np.testing.assert_array_equal(vs, order)
 
 
-@pytest.mark.parametrize("n", [0, 1, 100, 4096, 4097, 1_000_000, 16_777_216])
+@pytest.mark.parametrize("n", [0, 1, 100, 4096, 4097, 16_383, 16_384, 16_385, 1_000_000, 16_777_216])
 def test_exscan(ctx, n):
     print(f"exscan n={n}", flush=True)
     rng = np.random.default_rng(n)
@@ -98,6 +98,23 @@ def test_exscan(ctx, n):
     tot = ctx.alloc(4)
     _abi.call("onc_exscan_u32", ctx.h, bi.p, bo.p, n, tot.p)
     out = bo.download(np.uint32, n)
+    ref, rt = O.exscan(a)
+    np.testing.assert_array_equal(out, ref)
+    assert int(tot.download(np.uint32, 1)[0]) == rt
+
+
+@pytest.mark.parametrize("shift_in,shift_out", [(1, 0), (0, 3), (2, 1)])
+def test_exscan_unaligned_pointers(ctx, shift_in, shift_out):
+    """Input or output pointers that are not 16-byte aligned (the kernel then
+    moves one element at a time) give the same scan."""
+    n = 50_001
+    rng = np.random.default_rng(shift_in * 7 + shift_out)
+    a = rng.integers(0, 1 << 20, n).astype(np.uint32)
+    bi = ctx.upload(np.concatenate([np.zeros(shift_in, np.uint32), a]))
+    bo = ctx.alloc(4 * (n + shift_out))
+    tot = ctx.alloc(4)
+    _abi.call("onc_exscan_u32", ctx.h, bi.p.value + 4 * shift_in, bo.p.value + 4 * shift_out, n, tot.p)
+    out = bo.download(np.uint32, n + shift_out)[shift_out:]
     ref, rt = O.exscan(a)
     np.testing.assert_array_equal(out, ref)
     assert int(tot.download(np.uint32, 1)[0]) == rt

@@ -110,7 +110,8 @@ def test_capped_force_waves_give_the_same_bits(ctx, fused, waves):
 def test_set_tuning_rejects_bad_shapes(ctx):
     b = Batch(ctx)
     try:
-        for args in ((10, 4, 64, 4, 32), (16, 5, 64, 4, 32), (16, 4, 64, 3, 32), (16, 4, 64, 4, 0)):
+        for args in ((10, 4, 64, 4, 32), (16, 5, 64, 4, 32), (32, 4, 64, 4, 32), (16, 32, 64, 4, 32),
+                     (16, 4, 64, 3, 32), (16, 4, 64, 4, 0)):
             with pytest.raises(_abi.OncError):
                 b.set_tuning(*args)
         for f in (-1, 2):
