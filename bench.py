@@ -392,7 +392,8 @@ PROFILE_RUNS = {
 }
 KERNEL_PHASE = {"k_plan": "plan", "k_morton": "morton", "k_sort_pass": "sort", "k_permute_bins": "permute_bins",
                 "k_forces": "forces", "k_stencil": "stencil", "k_rule_tumor": "rule", "k_compact": "compact",
-                "k_forces_stencil": "forces_stencil", "k_rule_compact": "rule_compact", "k_copy_u32": "copy"}
+                "k_forces_stencil": "forces_stencil", "k_rule_compact": "rule_compact", "k_copy_u32": "copy",
+                "k_copy_v4": "copy"}
 COPY_BYTES = 4 * (64 << 20)  # one --calibrate copy launch reads and writes this many bytes
 
 
@@ -449,7 +450,7 @@ def summarize(d: str, workload: str = "large_1m_256", outdir: str = None):
                      f"{tot / 1e6:.2f} | {100.0 * tot / total:.1f} |")
     lines += ["", "HBM traffic per launch (two `--pmc` runs, one counter each):", ""]
     lines += ["    " + PROFILE_RUNS[c].format(d=d) for c in ("FETCH_SIZE", "WRITE_SIZE")]
-    lines += ["", f"Calibration: the plain copy kernel (`k_copy_u32`, {COPY_BYTES >> 20} MiB read + "
+    lines += ["", f"Calibration: the plain copy kernel (`k_copy_v4`, {COPY_BYTES >> 20} MiB read + "
                   f"{COPY_BYTES >> 20} MiB written per launch) reads FETCH_SIZE {pmc['copy']['FETCH_SIZE']:.0f} KiB "
                   f"and WRITE_SIZE {pmc['copy']['WRITE_SIZE']:.0f} KiB, so FETCH_SIZE is scaled by "
                   f"{cal['FETCH_SIZE']:.3f} and WRITE_SIZE by {cal['WRITE_SIZE']:.3f}.", "",
@@ -486,6 +487,9 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-kernels", action="store_true",
                     help="skip the pure-kernel rooflines (sort, scan, sweep, Morton at large sizes)")
+    ap.add_argument("--event-every", type=int, default=10,
+                    help="in the timed region, HIP events bracket the dominant kernel on every N-th step "
+                         "(an event pair between launches costs GPU time of its own)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run a plain 256 MiB copy kernel 5 times first (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -582,9 +586,10 @@ def main():
     dom = max((k for k in table if algorithmic_bytes(k, 1.0, 1, 1, 1, 1.0)), key=lambda k: table[k]["total_ms"])
 
     # timed region: steps 1..K of the workload from its seeded initial state,
-    # eager launches, HIP events around the dominant kernel only
+    # eager launches, HIP events around the dominant kernel on every
+    # --event-every-th step (its average launch time for the roofline)
     sim, batch = fresh()
-    batch.profile(True, phases=[dom])
+    batch.profile(True, phases=[dom], every=args.event_every)
     batch.sync()
     cu0 = cell_updates(sim)
     t0 = time.perf_counter()
@@ -650,7 +655,8 @@ def main():
             traffic = None
     roof = {"bound": "hbm", "kernel": dom, "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic,
-            "alg_bytes_per_launch": dom_bytes, "avg_launch_us": dom_avg}
+            "alg_bytes_per_launch": dom_bytes, "avg_launch_us": dom_avg,
+            "events_every_steps": args.event_every, "launches_timed": dom_n}
 
     kern = None if args.no_kernels else kernel_rooflines(ctx)
     cpu = None if args.no_cpu_baseline else cpu_baseline(w, args.cpu_steps, args.cpu_steps_1t)

@@ -213,6 +213,10 @@ int onc_batch_set_fused_rule(onc_batch* b, int fused);
    4 forces, 5 stencil, 6 rule, 7 compaction, 8 forces + stencil in one launch,
    9 rule + compaction in one launch); 0 disables */
 int onc_batch_profile(onc_batch* b, int phase_mask);
+/* the same, timing only every `every`-th eager step (the first, then every
+   every-th after it): each event pair costs GPU time between launches, so a
+   long run samples its steps instead (no counterpart in the reference) */
+int onc_batch_profile_every(onc_batch* b, int phase_mask, int every);
 #define ONC_PHASES 10
 int onc_batch_phase_times(onc_batch* b, double* us_total, uint64_t* launches, int n);
 

@@ -135,6 +135,8 @@ struct onc_batch {
     // profiling
     unsigned profile = 0;  // bit k: time phase k (onc_phase)
     bool prof_on = false;  // events are being recorded for the step being enqueued
+    int prof_every = 1;    // time every prof_every-th eager step
+    uint64_t prof_tick = 0;  // eager steps enqueued since profiling was set
     std::vector<unsigned> ev_mask;  // phases with both events recorded, per profiled step
     std::vector<hipEvent_t> ev;
     int ev_used = 0;
@@ -947,10 +949,11 @@ static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
     if (profiling) CKR(prof_reserve(b, nsteps));
     if (!use_graph) {
         for (int k = 0; k < nsteps; k++) {
-            b->prof_on = profiling;
-            if (profiling) b->ev_mask.push_back(0u);
+            const bool timed = profiling && (b->prof_tick++ % (uint64_t)b->prof_every) == 0u;
+            b->prof_on = timed;
+            if (timed) b->ev_mask.push_back(0u);
             const int rc = enqueue_step(b);
-            if (profiling) b->ev_used++;
+            if (timed) b->ev_used++;
             b->prof_on = false;
             CKR(rc);
         }
@@ -1120,12 +1123,19 @@ extern "C" int onc_batch_set_forces_waves(onc_batch* b, int waves) {
 }
 
 extern "C" int onc_batch_profile(onc_batch* b, int phase_mask) {
+    return onc_batch_profile_every(b, phase_mask, 1);
+}
+
+extern "C" int onc_batch_profile_every(onc_batch* b, int phase_mask, int every) {
     const int enable = phase_mask;
     if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
+    if (every < 1) return fail(ONC_ERR_INVALID, "every must be >= 1");
     CK(hipSetDevice(b->ctx->device));
     CK(hipStreamSynchronize(b->stream));
     b->ev_used = 0;
     b->ev_mask.clear();
+    b->prof_every = every;
+    b->prof_tick = 0;
     b->profile = (unsigned)enable & ((1u << ONC_PHASES) - 1u);
     for (int i = 0; i < ONC_PHASES; i++) {
         b->phase_us[i] = 0.0;

@@ -120,6 +120,7 @@ _PROTOS = {
     "onc_batch_set_fused_rule": (C.c_int, [_P, C.c_int]),
     "onc_batch_set_forces_waves": (C.c_int, [_P, C.c_int]),
     "onc_batch_profile": (C.c_int, [_P, C.c_int]),
+    "onc_batch_profile_every": (C.c_int, [_P, C.c_int, C.c_int]),
     "onc_batch_phase_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_int]),
     "onc_morton_keys": (C.c_int, [_P, _P, _P, _P, _u32, _i32p, C.c_float, C.c_int, _P]),
     "onc_sort_pairs": (C.c_int, [_P, _P, _P, _u32, C.c_int, _P, _P]),

@@ -167,9 +167,10 @@ class Batch:
     def sync(self) -> None:
         call("onc_batch_sync", self.h)
 
-    def profile(self, enable: bool = True, phases=None) -> None:
+    def profile(self, enable: bool = True, phases=None, every: int = 1) -> None:
         """Time phases of eager steps with HIP events: all of them, or only
-        the named ones (each event costs GPU time of its own)."""
+        the named ones (each event costs GPU time of its own), on every
+        `every`-th step."""
         if not enable:
             mask = 0
         elif phases is None:
@@ -178,7 +179,7 @@ class Batch:
             mask = 0
             for p in phases:
                 mask |= 1 << _abi.PHASE_NAMES.index(p)
-        call("onc_batch_profile", self.h, mask)
+        call("onc_batch_profile_every", self.h, mask, int(every))
 
     def phase_times(self) -> dict[str, tuple[float, int]]:
         us = (C.c_double * _abi.PHASES)()

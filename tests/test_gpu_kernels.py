@@ -200,3 +200,19 @@ def test_pure_kernel_timing(ctx):
         ctx.timing(False)
     with pytest.raises(_abi.OncError):
         ctx.last_kernel_us()
+
+
+@pytest.mark.parametrize("n,shift", [(1, 0), (3, 0), (4, 0), (5, 0), (1025, 0), (1_000_003, 0), (1_000_003, 1),
+                                     (4099, 2)])
+def test_copy_u32(ctx, n, shift):
+    """The plain copy (the byte counters' calibration kernel): 16-byte
+    streaming copies with a tail when the pointers are aligned, one word at a
+    time when one is shifted by `shift` words."""
+    rng = np.random.default_rng(n + shift)
+    a = rng.integers(0, 2**32, n + shift, dtype=np.uint64).astype(np.uint32)
+    src = ctx.upload(a)
+    dst = ctx.upload(np.full(n + 8, 0xDEADBEEF, dtype=np.uint32))
+    _abi.call("onc_copy_u32", ctx.h, src.p.value + 4 * shift, dst.p, n)
+    got = dst.download(np.uint32, n + 8)
+    np.testing.assert_array_equal(got[:n], a[shift:])
+    assert (got[n:] == 0xDEADBEEF).all()

@@ -196,3 +196,27 @@ def test_stop_step_freezes_a_sim_in_its_batch(ctx):
     b.close()
     for s in sims:
         s.close()
+
+
+def test_profile_every_samples_steps(ctx):
+    """Phase timing on every 3rd of 10 eager steps times steps 0, 3, 6, 9;
+    a stride below 1 is rejected."""
+    ws = _small_ensemble()[:2]
+    sims = [Sim(ctx, w.desc()) for w in ws]
+    b = Batch(ctx)
+    try:
+        for s, w in zip(sims, ws):
+            s.upload_cells(W.initial_cells(w))
+            b.add(s)
+        b.profile(True, phases=["plan", "forces_stencil"], every=3)
+        b.step(10, graph=False)
+        b.sync()
+        pt = b.phase_times()
+        assert pt["plan"][1] == 4 and pt["forces_stencil"][1] == 4
+        assert pt["plan"][0] > 0.0 and pt["morton"][1] == 0
+        with pytest.raises(_abi.OncError):
+            _abi.call("onc_batch_profile_every", b.h, 1, 0)
+    finally:
+        b.close()
+        for s in sims:
+            s.close()
