@@ -58,6 +58,13 @@ hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s);
 hipError_t onc_launch_morton(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_hist_only(const onc_kargs& a, int items, int grid, hipStream_t s);
 hipError_t onc_launch_sort_pass(const onc_kargs& a, int items, int pass, int grid, hipStream_t s);
+// the three-launch radix pass of large standalone sorts (tiles of 256 x ONC_SORT3_ITEMS keys)
+#define ONC_SORT3_ITEMS 16
+#define ONC_SORT3_MIN_N (1u << 20)
+hipError_t onc_launch_sort_up(const onc_u32* keys, onc_u32 n, onc_u32 shift, onc_u32* counts, onc_u32 T,
+                              hipStream_t s);
+hipError_t onc_launch_sort_down(const onc_u32* kin, const onc_u32* vin, onc_u32* kout, onc_u32* vout, onc_u32 n,
+                                onc_u32 shift, const onc_u32* offsets, onc_u32 T, hipStream_t s);
 hipError_t onc_launch_permute_bins(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_forces(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_stencil(const onc_kargs& a, onc_u32 nblocks, hipStream_t s);

@@ -518,6 +518,118 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Radix pass in three launches, for large standalone sorts: per-tile digit
+// counts (k_sort_up), an exclusive scan of the counts in digit-major order
+// (k_exscan; entry d * T + t becomes the output position of tile t's first
+// key of digit d), then rank and scatter (k_sort_down). No tile waits on
+// another: on a chip whose L2 is split across 8 XCDs every look-back round
+// trip goes through the memory-side cache, and a chain of them bounded the
+// one-launch pass. Costs one more read of the keys (4 B/key) and 8 B per
+// (tile, digit) of counts and offsets.
+// ---------------------------------------------------------------------------
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_sort_up(const onc_u32* __restrict__ keys, onc_u32 n, onc_u32 shift,
+                                                 onc_u32* __restrict__ counts, onc_u32 T) {
+    constexpr int TILE = 256 * ITEMS;
+    __shared__ onc_u32 s_h[256];
+    const int tid = threadIdx.x;
+    const onc_u32 t = blockIdx.x;
+    s_h[tid] = 0u;
+    __syncthreads();
+    const onc_u32 base = t * (onc_u32)TILE;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const onc_u32 i = base + (onc_u32)(k * 256 + tid);
+        if (i < n) atomicAdd(&s_h[(__builtin_nontemporal_load(keys + i) >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)tid * T + t] = s_h[tid];
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_sort_down(const onc_u32* __restrict__ kin, const onc_u32* __restrict__ vin,
+                                                   onc_u32* __restrict__ kout, onc_u32* __restrict__ vout, onc_u32 n,
+                                                   onc_u32 shift, const onc_u32* __restrict__ offsets, onc_u32 T) {
+    constexpr int TILE = 256 * ITEMS;
+    __shared__ onc_u32 s_keys[TILE];
+    __shared__ onc_u32 s_vals[TILE];
+    __shared__ onc_u32 s_wc[4 * 256];
+    __shared__ onc_u32 s_texcl[256];
+    __shared__ onc_u32 s_gbase[256];
+    __shared__ onc_u32 s_tmp[8];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const onc_u64 lt_mask = (1ull << lane) - 1ull;
+    const onc_u32 t = blockIdx.x;
+    const onc_u32 base = t * (onc_u32)TILE;
+    const onc_u32 cnt = min((onc_u32)TILE, n - base);
+    // this tile's output base per digit (thread = digit), loaded first
+    const onc_u32 gbase = offsets[(size_t)tid * T + t];
+    for (int i = tid; i < 4 * 256; i += 256) s_wc[i] = 0u;
+    onc_u32 key[ITEMS], val[ITEMS], rnk[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const onc_u32 idx = (onc_u32)(w * 64 * ITEMS + k * 64 + lane);
+        key[k] = 0u;
+        val[k] = 0u;
+        if (idx < cnt) {
+            key[k] = __builtin_nontemporal_load(kin + base + idx);
+            val[k] = __builtin_nontemporal_load(vin + base + idx);
+        }
+    }
+    __syncthreads();
+    // rank inside the wave, items in (k, lane) order = input order
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const onc_u32 idx = (onc_u32)(w * 64 * ITEMS + k * 64 + lane);
+        const bool valid = idx < cnt;
+        const onc_u32 d = (key[k] >> shift) & 255u;
+        onc_u64 m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const onc_u64 bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const onc_u32 pre = s_wc[w * 256 + d];
+        const onc_u32 r = (onc_u32)__popcll(m & lt_mask);
+        rnk[k] = pre + r;
+        if (valid && r == 0u) s_wc[w * 256 + d] = pre + (onc_u32)__popcll(m);
+    }
+    __syncthreads();
+    const onc_u32 c0 = s_wc[tid], c1 = s_wc[256 + tid], c2 = s_wc[512 + tid], c3 = s_wc[768 + tid];
+    s_wc[tid] = 0u;
+    s_wc[256 + tid] = c0;
+    s_wc[512 + tid] = c0 + c1;
+    s_wc[768 + tid] = c0 + c1 + c2;
+    s_texcl[tid] = block_exscan256(c0 + c1 + c2 + c3, s_tmp, nullptr);
+    s_gbase[tid] = gbase;
+    __syncthreads();
+    // reorder through LDS by digit, then scatter runs of equal digits
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const onc_u32 idx = (onc_u32)(w * 64 * ITEMS + k * 64 + lane);
+        if (idx < cnt) {
+            const onc_u32 d = (key[k] >> shift) & 255u;
+            const onc_u32 pos = s_texcl[d] + s_wc[w * 256 + d] + rnk[k];
+            s_keys[pos] = key[k];
+            s_vals[pos] = val[k];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const onc_u32 j = (onc_u32)(k * 256 + tid);
+        if (j < cnt) {
+            const onc_u32 kk = s_keys[j];
+            const onc_u32 d = (kk >> shift) & 255u;
+            const onc_u32 o = s_gbase[d] + (j - s_texcl[d]);
+            kout[o] = kk;
+            vout[o] = s_vals[j];
+        }
+    }
+}
+
 // Fixed (Dirichlet) voxel predicate (engine/assets/shaders/tumor_sim.wgsl:420-423).
 __device__ __forceinline__ bool onc_is_fixed(const onc_dfield& f, const int* d, int x, int y, int z) {
     if (f.bc == 0) return x == 0 || y == 0 || z == 0 || x == d[0] - 1 || y == d[1] - 1 || z == d[2] - 1;
@@ -1576,6 +1688,17 @@ hipError_t onc_launch_philox(const onc_u32* ctr, onc_u32 n, onc_u32 k0, onc_u32 
                              hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_philox, dim3(grid_for(n)), dim3(256), 0, s, ctr, n, k0, k1, out);
+    return hipGetLastError();
+}
+hipError_t onc_launch_sort_up(const onc_u32* keys, onc_u32 n, onc_u32 shift, onc_u32* counts, onc_u32 T,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_up<ONC_SORT3_ITEMS>, dim3(T), dim3(256), 0, s, keys, n, shift, counts, T);
+    return hipGetLastError();
+}
+hipError_t onc_launch_sort_down(const onc_u32* kin, const onc_u32* vin, onc_u32* kout, onc_u32* vout, onc_u32 n,
+                                onc_u32 shift, const onc_u32* offsets, onc_u32 T, hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_down<ONC_SORT3_ITEMS>, dim3(T), dim3(256), 0, s, kin, vin, kout, vout, n, shift,
+                       offsets, T);
     return hipGetLastError();
 }
 hipError_t onc_launch_exscan(const onc_u32* in, onc_u32* out, onc_u32 n, onc_u32* total, onc_u32* status,

@@ -1182,6 +1182,54 @@ extern "C" int onc_morton_keys(onc_ctx* c, const float* px, const float* py, con
     return tread(c);
 }
 
+// Large standalone sorts: each pass as counts -> scan of the counts -> rank
+// and scatter (three launches, no look-back chain between tiles).
+static int sort_pairs_3launch(onc_ctx* c, const uint32_t* kin, const uint32_t* vin, uint32_t n, int passes,
+                              uint32_t* kout, uint32_t* vout) {
+    const size_t T = (n + 256u * ONC_SORT3_ITEMS - 1) / (256u * ONC_SORT3_ITEMS);
+    const size_t nc = 256u * T;  // (digit, tile) counts
+    const size_t stiles = (nc + ONC_SCAN_TILE - 1) / ONC_SCAN_TILE;
+    if (nc > 0xFFFFFFFFull) return fail(ONC_ERR_INVALID, "too many sort tiles");
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_k0 = take(n * 4ull), o_k1 = take(n * 4ull), o_v0 = take(n * 4ull), o_v1 = take(n * 4ull);
+    const size_t o_cnt = take(nc * 4), o_off = take(nc * 4);
+    const size_t o_ss = take((size_t)passes * stiles * 8), o_err = take(256);
+    void* base;
+    CKR(ctx_scratch(c, off, &base));
+    char* B = (char*)base;
+    onc_u32* keys[2] = {(onc_u32*)(B + o_k0), (onc_u32*)(B + o_k1)};
+    onc_u32* vals[2] = {(onc_u32*)(B + o_v0), (onc_u32*)(B + o_v1)};
+    onc_u32* counts = (onc_u32*)(B + o_cnt);
+    onc_u32* offsets = (onc_u32*)(B + o_off);
+    onc_u32* err = (onc_u32*)(B + o_err);
+    hipStream_t st = c->stream;
+    CK(hipMemsetAsync(B + o_ss, 0, (size_t)passes * stiles * 8 + 256, st));
+    CK(hipMemcpyAsync(keys[0], kin, n * 4ull, hipMemcpyDeviceToDevice, st));
+    CK(hipMemcpyAsync(vals[0], vin, n * 4ull, hipMemcpyDeviceToDevice, st));
+    tmark(c, 0);
+    for (int p = 0; p < passes; p++) {
+        const int i = p & 1;
+        CK(onc_launch_sort_up(keys[i], n, 8u * (onc_u32)p, counts, (onc_u32)T, st));
+        CK(onc_launch_exscan(counts, offsets, (onc_u32)nc, nullptr, (onc_u32*)(B + o_ss + (size_t)p * stiles * 8),
+                             err, st));
+        CK(onc_launch_sort_down(keys[i], vals[i], keys[i ^ 1], vals[i ^ 1], n, 8u * (onc_u32)p, offsets,
+                                (onc_u32)T, st));
+    }
+    tmark(c, 1);
+    CK(hipMemcpyAsync(kout, keys[passes & 1], n * 4ull, hipMemcpyDeviceToDevice, st));
+    CK(hipMemcpyAsync(vout, vals[passes & 1], n * 4ull, hipMemcpyDeviceToDevice, st));
+    onc_u32 herr = 0;
+    CK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (herr) return fail(ONC_ERR_DEVICE, "scan look-back wait expired");
+    return tread(c);
+}
+
 extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* vin, uint32_t n, int key_bits,
                               uint32_t* kout, uint32_t* vout) {
     if (!c || (n && (!kin || !vin || !kout || !vout))) return fail(ONC_ERR_INVALID, "NULL argument");
@@ -1189,6 +1237,7 @@ extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* v
     if (n > (1u << 30)) return fail(ONC_ERR_INVALID, "n > 2^30");
     CK(hipSetDevice(c->device));
     if (n == 0) return ONC_OK;
+    if (n >= ONC_SORT3_MIN_N) return sort_pairs_3launch(c, kin, vin, n, (key_bits + 7) / 8, kout, vout);
     const int items = 16;
     const size_t tile = 256 * items;
     const size_t tiles = (n + tile - 1) / tile;
