@@ -21,7 +21,7 @@
 #define ONC_DERR_SCAN 4u
 #define ONC_DERR_BIG_BIN 8u
 #define ONC_MAX_BIN_CELLS 65535u
-#define ONC_SCAN_TILE 16384u  // elements per tile of the standalone exclusive scan
+#define ONC_SCAN_MIN_TILE 4096u  // fewest elements per tile of the exclusive scan (status words to allocate)
 
 // Width in float4 columns of a float4 stencil tile: the smallest power of two
 // >= X/4, capped at sbx (a power of two <= 256). Lanes of one row segment of

@@ -1501,13 +1501,14 @@ __global__ __launch_bounds__(256) void k_fill_u32(onc_u32* p, onc_u32 v, size_t 
 // so every load and store instruction of a wave covers 1 KB in one piece
 // and no LDS transpose is needed. Per group the wave scans the lane sums
 // with shuffles; four wave totals and one look-back give the tile's
-// offsets. Large tiles (16384 elements) keep the look-backs few: the scan
-// is bound by the latency of the look-back chain, not by the bytes moved,
-// once tiles are small. Streaming loads and stores: every element is read
+// offsets. Large tiles (16K or 32K elements) keep the look-backs few: the
+// scan is bound by the latency of the look-back chain, not by the bytes
+// moved, once tiles are small. Streaming loads and stores: every element is read
 // and written once. V4 = false reads and writes the same groups one element
-// at a time (pointers that are not 16-byte aligned).
-template <int ITEMS, bool V4>
-__global__ __launch_bounds__(256) void k_exscan(const onc_u32* __restrict__ in, onc_u32* __restrict__ out,
+// at a time (pointers that are not 16-byte aligned), as does a partial last
+// tile.
+template <int ITEMS, bool V4, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_exscan(const onc_u32* __restrict__ in, onc_u32* __restrict__ out,
                                                 onc_u32 n, onc_u32* total, onc_u64* status, onc_u32* err) {
     // one block per tile, in static order: a tile's look-back waits only on
     // lower tiles, whose blocks were dispatched before it (a shared tile
@@ -1517,30 +1518,35 @@ __global__ __launch_bounds__(256) void k_exscan(const onc_u32* __restrict__ in, 
     __shared__ onc_u32 s_w[4];
     __shared__ onc_u32 s_pre;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const onc_u32 ntiles = (n + TILE - 1) / TILE;
+    const onc_u32 ntiles = (onc_u32)(((size_t)n + TILE - 1) / TILE);
     const onc_u32 g = blockIdx.x;
     if (g >= ntiles) return;
     const onc_u32 lim = n - g * (onc_u32)TILE;  // elements from the tile start to the end
     const onc_u32 wbase = (onc_u32)w * (onc_u32)(64 * ITEMS) + (onc_u32)lane * 4u;
     const onc_u32* __restrict__ src = in + (size_t)g * TILE;
     onc_u32* __restrict__ dst = out + (size_t)g * TILE;
+    typedef onc_u32 v4u __attribute__((ext_vector_type(4)));
+    const bool full = V4 && lim >= (onc_u32)TILE;  // every group whole: unguarded 16-byte accesses
     uint4 x[K];
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        const onc_u32 e = wbase + (onc_u32)(k * 256);
-        if (V4 && e + 4u <= lim) {
-            typedef onc_u32 v4u __attribute__((ext_vector_type(4)));
-            const v4u t = __builtin_nontemporal_load((const v4u*)(src + e));
+        for (int k = 0; k < K; k++) {
+            const v4u t = __builtin_nontemporal_load((const v4u*)(src + wbase + k * 256));
             x[k] = make_uint4(t.x, t.y, t.z, t.w);
-        } else {
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const onc_u32 e = wbase + (onc_u32)(k * 256);
             x[k].x = e < lim ? __builtin_nontemporal_load(src + e) : 0u;
             x[k].y = e + 1u < lim ? __builtin_nontemporal_load(src + e + 1u) : 0u;
             x[k].z = e + 2u < lim ? __builtin_nontemporal_load(src + e + 2u) : 0u;
             x[k].w = e + 3u < lim ? __builtin_nontemporal_load(src + e + 3u) : 0u;
         }
     }
-    // exclusive prefix of each group inside the wave's run
-    onc_u32 gex[K];
+    // each group becomes the exclusive prefixes of its four elements inside
+    // the wave's run (in place: no registers beyond the tile itself, so four
+    // waves fit per SIMD)
     onc_u32 wtot = 0u;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -1551,7 +1557,8 @@ __global__ __launch_bounds__(256) void k_exscan(const onc_u32* __restrict__ in, 
             const onc_u32 t = __shfl_up(y, o, 64);
             if (lane >= o) y += t;
         }
-        gex[k] = wtot + y - v;
+        const onc_u32 e0 = wtot + y - v, e1 = e0 + x[k].x, e2 = e1 + x[k].y, e3 = e2 + x[k].z;
+        x[k] = make_uint4(e0, e1, e2, e3);
         wtot += __shfl(y, 63, 64);
     }
     if (lane == 0) s_w[w] = wtot;
@@ -1580,26 +1587,20 @@ __global__ __launch_bounds__(256) void k_exscan(const onc_u32* __restrict__ in, 
     }
     __syncthreads();
     const onc_u32 off = s_pre + before;
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        const onc_u32 e = wbase + (onc_u32)(k * 256);
-        onc_u32 r = off + gex[k];
-        const onc_u32 o0 = r;
-        r += x[k].x;
-        const onc_u32 o1 = r;
-        r += x[k].y;
-        const onc_u32 o2 = r;
-        r += x[k].z;
-        const onc_u32 o3 = r;
-        if (V4 && e + 4u <= lim) {
-            typedef onc_u32 v4u __attribute__((ext_vector_type(4)));
-            const v4u t = {o0, o1, o2, o3};
-            __builtin_nontemporal_store(t, (v4u*)(dst + e));
-        } else {
-            if (e < lim) __builtin_nontemporal_store(o0, dst + e);
-            if (e + 1u < lim) __builtin_nontemporal_store(o1, dst + e + 1u);
-            if (e + 2u < lim) __builtin_nontemporal_store(o2, dst + e + 2u);
-            if (e + 3u < lim) __builtin_nontemporal_store(o3, dst + e + 3u);
+        for (int k = 0; k < K; k++) {
+            const v4u t = {off + x[k].x, off + x[k].y, off + x[k].z, off + x[k].w};
+            __builtin_nontemporal_store(t, (v4u*)(dst + wbase + k * 256));
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const onc_u32 e = wbase + (onc_u32)(k * 256);
+            if (e < lim) __builtin_nontemporal_store(off + x[k].x, dst + e);
+            if (e + 1u < lim) __builtin_nontemporal_store(off + x[k].y, dst + e + 1u);
+            if (e + 2u < lim) __builtin_nontemporal_store(off + x[k].z, dst + e + 2u);
+            if (e + 3u < lim) __builtin_nontemporal_store(off + x[k].w, dst + e + 3u);
         }
     }
 }
@@ -1704,13 +1705,20 @@ hipError_t onc_launch_sort_down(const onc_u32* kin, const onc_u32* vin, onc_u32*
 hipError_t onc_launch_exscan(const onc_u32* in, onc_u32* out, onc_u32 n, onc_u32* total, onc_u32* status,
                              onc_u32* err, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    const onc_u32 tiles = (n + ONC_SCAN_TILE - 1u) / ONC_SCAN_TILE;
-    if ((((uintptr_t)in) | ((uintptr_t)out)) & 15u)
-        hipLaunchKernelGGL((k_exscan<ONC_SCAN_TILE / 256, false>), dim3(tiles), dim3(256), 0, s, in, out, n, total,
+    // 32K-element tiles (235 VGPRs, two waves per SIMD) once there are at
+    // least 512 of them, 16K-element tiles (124 VGPRs, four waves per SIMD)
+    // below that; pointers that are not 16-byte aligned take 4K-element
+    // tiles of one-element accesses
+    if ((((uintptr_t)in) | ((uintptr_t)out)) & 15u) {
+        hipLaunchKernelGGL((k_exscan<16, false, 1>), dim3((unsigned)(((size_t)n + 4095u) / 4096u)), dim3(256), 0, s, in, out, n, total,
                            (onc_u64*)status, err);
-    else
-        hipLaunchKernelGGL((k_exscan<ONC_SCAN_TILE / 256, true>), dim3(tiles), dim3(256), 0, s, in, out, n, total,
+    } else if (n >= (1u << 24)) {
+        hipLaunchKernelGGL((k_exscan<128, true, 2>), dim3((unsigned)(((size_t)n + 32767u) / 32768u)), dim3(256), 0, s, in, out, n,
+                           total, (onc_u64*)status, err);
+    } else {
+        hipLaunchKernelGGL((k_exscan<64, true, 4>), dim3((unsigned)(((size_t)n + 16383u) / 16384u)), dim3(256), 0, s, in, out, n, total,
                            (onc_u64*)status, err);
+    }
     return hipGetLastError();
 }
 hipError_t onc_launch_fill_u32(onc_u32* p, onc_u32 v, size_t n, hipStream_t s) {

@@ -1188,7 +1188,7 @@ static int sort_pairs_3launch(onc_ctx* c, const uint32_t* kin, const uint32_t* v
                               uint32_t* kout, uint32_t* vout) {
     const size_t T = (n + 256u * ONC_SORT3_ITEMS - 1) / (256u * ONC_SORT3_ITEMS);
     const size_t nc = 256u * T;  // (digit, tile) counts
-    const size_t stiles = (nc + ONC_SCAN_TILE - 1) / ONC_SCAN_TILE;
+    const size_t stiles = (nc + ONC_SCAN_MIN_TILE - 1) / ONC_SCAN_MIN_TILE;
     if (nc > 0xFFFFFFFFull) return fail(ONC_ERR_INVALID, "too many sort tiles");
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -1319,7 +1319,7 @@ extern "C" int onc_exscan_u32(onc_ctx* c, const uint32_t* in, uint32_t* out, uin
         CK(hipStreamSynchronize(c->stream));
         return ONC_OK;
     }
-    const size_t tiles = (n + ONC_SCAN_TILE - 1) / ONC_SCAN_TILE;
+    const size_t tiles = (n + ONC_SCAN_MIN_TILE - 1) / ONC_SCAN_MIN_TILE;
     void* base;
     CKR(ctx_scratch(c, tiles * 8 + 256, &base));
     onc_u64* status = (onc_u64*)base;

@@ -86,7 +86,8 @@ def test_sort_morton_like_keys_many_duplicates(ctx):
     np.testing.assert_array_equal(vs, order)
 
 
-@pytest.mark.parametrize("n", [0, 1, 100, 4096, 4097, 16_383, 16_384, 16_385, 1_000_000, 16_777_216])
+@pytest.mark.parametrize("n", [0, 1, 100, 4096, 4097, 16_383, 16_384, 16_385, 1_000_000, 16_777_215,
+                               16_777_216, 16_809_985])
 def test_exscan(ctx, n):
     print(f"exscan n={n}", flush=True)
     rng = np.random.default_rng(n)
