@@ -526,7 +526,11 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
 // another: on a chip whose L2 is split across 8 XCDs every look-back round
 // trip goes through the memory-side cache, and a chain of them bounded the
 // one-launch pass. Costs one more read of the keys (4 B/key) and 8 B per
-// (tile, digit) of counts and offsets.
+// (tile, digit) of counts and offsets. With no waits between tiles, each XCD
+// takes a contiguous range of tiles: the counts and offsets a digit row holds
+// for neighbouring tiles then share cache lines in one L2 (instead of eight
+// partial copies of each line), and so do the neighbouring runs of a digit
+// in the scatter.
 // ---------------------------------------------------------------------------
 template <int ITEMS>
 __global__ __launch_bounds__(256) void k_sort_up(const onc_u32* __restrict__ keys, onc_u32 n, onc_u32 shift,
@@ -534,14 +538,27 @@ __global__ __launch_bounds__(256) void k_sort_up(const onc_u32* __restrict__ key
     constexpr int TILE = 256 * ITEMS;
     __shared__ onc_u32 s_h[256];
     const int tid = threadIdx.x;
-    const onc_u32 t = blockIdx.x;
+    const onc_u32 t = onc_xcd_block(blockIdx.x, gridDim.x);  // each XCD a contiguous range of tiles
     s_h[tid] = 0u;
     __syncthreads();
     const onc_u32 base = t * (onc_u32)TILE;
+    if (base + (onc_u32)TILE <= n) {
+        // whole tile: 16-byte loads (the key buffers are 256-byte aligned)
+        typedef onc_u32 v4u __attribute__((ext_vector_type(4)));
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        const onc_u32 i = base + (onc_u32)(k * 256 + tid);
-        if (i < n) atomicAdd(&s_h[(__builtin_nontemporal_load(keys + i) >> shift) & 255u], 1u);
+        for (int k = 0; k < ITEMS / 4; k++) {
+            const v4u q = __builtin_nontemporal_load((const v4u*)(keys + base) + k * 256 + tid);
+            atomicAdd(&s_h[(q.x >> shift) & 255u], 1u);
+            atomicAdd(&s_h[(q.y >> shift) & 255u], 1u);
+            atomicAdd(&s_h[(q.z >> shift) & 255u], 1u);
+            atomicAdd(&s_h[(q.w >> shift) & 255u], 1u);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const onc_u32 i = base + (onc_u32)(k * 256 + tid);
+            if (i < n) atomicAdd(&s_h[(__builtin_nontemporal_load(keys + i) >> shift) & 255u], 1u);
+        }
     }
     __syncthreads();
     counts[(size_t)tid * T + t] = s_h[tid];
@@ -560,7 +577,7 @@ __global__ __launch_bounds__(256) void k_sort_down(const onc_u32* __restrict__ k
     __shared__ onc_u32 s_tmp[8];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const onc_u64 lt_mask = (1ull << lane) - 1ull;
-    const onc_u32 t = blockIdx.x;
+    const onc_u32 t = onc_xcd_block(blockIdx.x, gridDim.x);  // each XCD a contiguous range of tiles
     const onc_u32 base = t * (onc_u32)TILE;
     const onc_u32 cnt = min((onc_u32)TILE, n - base);
     // this tile's output base per digit (thread = digit), loaded first
