@@ -395,6 +395,21 @@ KERNEL_PHASE = {"k_plan": "plan", "k_morton": "morton", "k_sort_pass": "sort", "
                 "k_forces_stencil": "forces_stencil", "k_rule_compact": "rule_compact", "k_copy_u32": "copy",
                 "k_copy_v4": "copy"}
 COPY_BYTES = 4 * (64 << 20)  # one --calibrate copy launch reads and writes this many bytes
+# the pure kernels alone (bench.py --kernels-only): trace and byte counters
+PROFILE_RUNS_KERNELS = {
+    "kt": "rocprofv3 --kernel-trace --stats -d {d}/kkt -o run -- python3 bench.py --kernels-only",
+    "FETCH_SIZE": "rocprofv3 --pmc FETCH_SIZE -d {d}/kFETCH_SIZE -o run -- python3 bench.py --kernels-only --calibrate",
+    "WRITE_SIZE": "rocprofv3 --pmc WRITE_SIZE -d {d}/kWRITE_SIZE -o run -- python3 bench.py --kernels-only --calibrate",
+}
+# algorithmic bytes per dispatch of the pure kernels at the sizes kernel_rooflines runs
+PURE_ALG_BYTES = {
+    "k_copy_v4": (8 * (64 << 20), "256 MiB read + 256 MiB written"),
+    "k_sort_up": (4 * (16 << 20) + 4 * 256 * 4096, "16M keys read, 256 x 4096 counts written"),
+    "k_sort_down": (16 * (16 << 20) + 4 * 256 * 4096, "16M (key, value) pairs read and written, 1M offsets read"),
+    "k_exscan": (None, "8 B per element"),
+    "k_stencil": (8 * 512 ** 3, "512^3 binary32 read + written"),
+    "k_morton_keys": (16 * (16 << 20), "16M positions (12 B) read, keys (4 B) written"),
+}
 
 
 def _kernel_phase(name: str):
@@ -464,9 +479,58 @@ def summarize(d: str, workload: str = "large_1m_256", outdir: str = None):
     print(f"wrote {tj} and {md}")
 
 
+def summarize_kernels(d: str, outdir: str = None):
+    """Reduce the PROFILE_RUNS_KERNELS outputs under `d` to
+    profiles/pure_kernels_16m.md: per pure kernel, its traced duration and
+    its HBM bytes per dispatch from the two calibrated counter runs, against
+    its algorithmic bytes."""
+    kt = {name.split("(")[0]: (n, avg) for name, n, avg in _rocpd(os.path.join(d, "kkt")).execute(
+        "select name, count(*), avg(end-start) from kernels group by name").fetchall()}
+    pmc = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        for name, v in _rocpd(os.path.join(d, "k" + ctr)).execute(
+                "select kernel_name, avg(value) from counters_collection where counter_name = ? "
+                "group by kernel_name", (ctr,)).fetchall():
+            pmc.setdefault(name.split("(")[0], {})[ctr] = float(v)
+    copy = pmc["k_copy_v4"]
+    cal = {c: COPY_BYTES / 1024.0 / copy[c] for c in ("FETCH_SIZE", "WRITE_SIZE")}
+    lines = ["# Pure kernels at large sizes: duration and HBM traffic per dispatch", "",
+             "Kernel trace, then two counter runs (one counter each) of the same kernels:", ""]
+    lines += ["    export TMPDIR=/tmp; " + PROFILE_RUNS_KERNELS[c].format(d=d) for c in ("kt", "FETCH_SIZE", "WRITE_SIZE")]
+    lines += ["", f"Calibration: the plain copy (`k_copy_v4`, {COPY_BYTES >> 20} MiB each way) reads FETCH_SIZE "
+                  f"{copy['FETCH_SIZE']:.0f} KiB and WRITE_SIZE {copy['WRITE_SIZE']:.0f} KiB, so FETCH_SIZE is scaled "
+                  f"by {cal['FETCH_SIZE']:.3f} and WRITE_SIZE by {cal['WRITE_SIZE']:.3f}.", "",
+              "`k_sort_up` / `k_sort_down` are one radix pass each (16M pairs); `k_exscan<64, ...>` is the scan of a "
+              "pass's 1M (digit, tile) counts, `k_exscan<128, ...>` the 64M-element scan.", "",
+              "| kernel | calls | avg us | HBM read MB | HBM written MB | algorithmic MB | traffic / algorithmic |",
+              "|---|---|---|---|---|---|---|"]
+    for name in sorted(pmc):
+        v = pmc[name]
+        if len(v) != 2 or name not in kt:
+            continue
+        rd = v["FETCH_SIZE"] * cal["FETCH_SIZE"] * 1024.0
+        wr = v["WRITE_SIZE"] * cal["WRITE_SIZE"] * 1024.0
+        base = name.split("<")[0].replace("void ", "")
+        alg = PURE_ALG_BYTES.get(base, (None, ""))[0]
+        if base == "k_exscan":
+            alg = 8.0 * (64 << 20) if "<128" in name else 8.0 * 256 * 4096
+        n, avg = kt[name]
+        lines.append(f"| `{name}` | {n} | {avg / 1e3:.1f} | {rd / 1e6:.1f} | {wr / 1e6:.1f} | "
+                     + (f"{alg / 1e6:.1f} | {(rd + wr) / alg:.2f} |" if alg else "— | — |"))
+    outdir = outdir or os.path.join(ROOT, "profiles")
+    os.makedirs(outdir, exist_ok=True)
+    md = os.path.join(outdir, "pure_kernels_16m.md")
+    with open(md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(f"wrote {md}")
+
+
 def main():
     if len(sys.argv) >= 3 and sys.argv[1] == "--summarize":
         summarize(sys.argv[2], outdir=sys.argv[3] if len(sys.argv) > 3 else None)
+        return
+    if len(sys.argv) >= 3 and sys.argv[1] == "--summarize-kernels":
+        summarize_kernels(sys.argv[2], outdir=sys.argv[3] if len(sys.argv) > 3 else None)
         return
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -490,6 +554,8 @@ def main():
     ap.add_argument("--event-every", type=int, default=10,
                     help="in the timed region, HIP events bracket the dominant kernel on every N-th step "
                          "(an event pair between launches costs GPU time of its own)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="run only the pure-kernel rooflines (one JSON line with their figures)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run a plain 256 MiB copy kernel 5 times first (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -519,6 +585,10 @@ def main():
             _abi.call("onc_copy_u32", ctx.h, a.p, b.p, nw)
         a.free()
         b.free()
+
+    if args.kernels_only:
+        print(json.dumps({"kernels": kernel_rooflines(ctx)}))
+        return
 
     w = W.large_c2(n_cells=args.cells)
     desc = w.desc()
