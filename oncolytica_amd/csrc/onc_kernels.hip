@@ -569,8 +569,8 @@ __global__ __launch_bounds__(256) void k_sort_down(const onc_u32* __restrict__ k
                                                    onc_u32* __restrict__ kout, onc_u32* __restrict__ vout, onc_u32 n,
                                                    onc_u32 shift, const onc_u32* __restrict__ offsets, onc_u32 T) {
     constexpr int TILE = 256 * ITEMS;
-    __shared__ onc_u32 s_keys[TILE];
-    __shared__ onc_u32 s_vals[TILE];
+    __shared__ __attribute__((aligned(16))) onc_u32 s_keys[TILE];
+    __shared__ __attribute__((aligned(16))) onc_u32 s_vals[TILE];
     __shared__ onc_u32 s_wc[4 * 256];
     __shared__ onc_u32 s_texcl[256];
     __shared__ onc_u32 s_gbase[256];
@@ -584,14 +584,35 @@ __global__ __launch_bounds__(256) void k_sort_down(const onc_u32* __restrict__ k
     const onc_u32 gbase = offsets[(size_t)tid * T + t];
     for (int i = tid; i < 4 * 256; i += 256) s_wc[i] = 0u;
     onc_u32 key[ITEMS], val[ITEMS], rnk[ITEMS];
+    if (cnt == (onc_u32)TILE) {
+        // whole tile: 16-byte streaming loads into LDS in input order (the
+        // pair buffers are 256-byte aligned), then each thread's items back
+        typedef onc_u32 v4u __attribute__((ext_vector_type(4)));
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        const onc_u32 idx = (onc_u32)(w * 64 * ITEMS + k * 64 + lane);
-        key[k] = 0u;
-        val[k] = 0u;
-        if (idx < cnt) {
-            key[k] = __builtin_nontemporal_load(kin + base + idx);
-            val[k] = __builtin_nontemporal_load(vin + base + idx);
+        for (int k = 0; k < ITEMS / 4; k++) {
+            const int e = (k * 256 + tid) * 4;
+            const v4u a = __builtin_nontemporal_load((const v4u*)(kin + base + e));
+            const v4u b = __builtin_nontemporal_load((const v4u*)(vin + base + e));
+            *(uint4*)&s_keys[e] = make_uint4(a.x, a.y, a.z, a.w);
+            *(uint4*)&s_vals[e] = make_uint4(b.x, b.y, b.z, b.w);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const int idx = w * 64 * ITEMS + k * 64 + lane;
+            key[k] = s_keys[idx];
+            val[k] = s_vals[idx];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const onc_u32 idx = (onc_u32)(w * 64 * ITEMS + k * 64 + lane);
+            key[k] = 0u;
+            val[k] = 0u;
+            if (idx < cnt) {
+                key[k] = __builtin_nontemporal_load(kin + base + idx);
+                val[k] = __builtin_nontemporal_load(vin + base + idx);
+            }
         }
     }
     __syncthreads();
