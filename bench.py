@@ -180,6 +180,49 @@ def kernel_rooflines(ctx):
     return out
 
 
+def metrics_leg(ctx, desc, cols0, Sim):
+    """Metric rules on the device (onc_sim_metrics): the benchmark model's
+    translated metric rule (population, summed age, summed O2 at the cells,
+    cells ready to divide) over the 1M committed initial cells, ten calls
+    timed on the host clock (each includes its commit check, two launches and
+    the 16-byte read-back). Algorithmic bytes: position and age (16 B) plus
+    one O2 gather (4 B) per cell."""
+    from oncolytica_amd.hip._sim import Module
+    from oncolytica_amd.hip._translator import CellSchema, TissueSchema, Translator, param_values
+    from oncolytica_amd.spheroid import Spheroid, SpheroidMetrics, SpheroidParams, SpheroidTissue, TumorCell
+    import numpy as np
+    model = Spheroid()
+    tr = Translator(model, CellSchema.of(TumorCell), TissueSchema.of(SpheroidTissue), model._params)
+    src = tr.translate([model.lifecycle], [model.count], SpheroidMetrics)
+    t0 = time.perf_counter()
+    mod = Module(ctx, src)
+    t_build = time.perf_counter() - t0
+    sim = Sim(ctx, desc)
+    sim.upload_cells(cols0)
+    # the workload's rule parameters (built-in order: division age, division
+    # O2, hypoxia, drug kill, daughter offset) in the translated module's slots
+    a0, a1, a2, a3, a4 = (float(x) for x in desc.params[:5])
+    sim.set_params(param_values(SpheroidParams(div_age=a0, div_o2=a1, hypoxia=a2, drug_kill=a3, daughter_offset=a4)))
+    nv = len(tr.metric_fields)
+    sim.metrics(mod, nv)
+    ts = []
+    for _ in range(10):
+        t0 = time.perf_counter()
+        words = sim.metrics(mod, nv)
+        ts.append(time.perf_counter() - t0)
+    n = len(cols0[0])
+    vals = {}
+    for (name, t), w in zip(tr.metric_fields, words):
+        vals[name] = float(np.array([w], np.uint32).view(np.float32)[0]) if t == "f32" else int(w)
+    us = float(np.median(ts)) * 1e6
+    sim.close()
+    mod.close()
+    out = {"cells": n, "us_per_call": us, "alg_bytes": 20 * n, "GBps": 20 * n / (us * 1e-6) / 1e9,
+           "hiprtc_build_s": t_build, "values": vals}
+    _log(f"metrics: {n:,} cells in {us:.1f} us per call ({out['GBps']:.0f} GB/s incl. host sync), {vals}")
+    return out
+
+
 def ensemble(ctx, n_sims: int, steps: int):
     """Batched uneven ensemble (64 sims, cells log-uniform in [10k, 500k],
     grids from {64, 128, 192}^3, per-sim uptake / division age / hypoxia):
@@ -728,6 +771,7 @@ def main():
             "alg_bytes_per_launch": dom_bytes, "avg_launch_us": dom_avg,
             "events_every_steps": args.event_every, "launches_timed": dom_n}
 
+    met = metrics_leg(ctx, desc, cols0, Sim)
     kern = None if args.no_kernels else kernel_rooflines(ctx)
     cpu = None if args.no_cpu_baseline else cpu_baseline(w, args.cpu_steps, args.cpu_steps_1t)
     ens = ensemble(ctx, args.ensemble_sims, args.ensemble_steps) if args.ensemble_sims > 0 else None
@@ -757,6 +801,7 @@ def main():
         "phases": table,
         "phases_separate_launches": table_sep,
         "kernels": kern,
+        "metrics": met,
         "value_separate_launches": sep_value,
         "cpu_baseline": cpu,
         "ensemble": ens,

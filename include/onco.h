@@ -173,6 +173,16 @@ int onc_sim_counts(onc_sim* sim, uint32_t* n, uint32_t* next_id, uint32_t* steps
    batch hold jobs with different horizons without host round trips).
    Default: no limit (0xFFFFFFFF). */
 int onc_sim_set_stop(onc_sim* sim, uint32_t stop);
+/* metric rules: runs the module's onc_metric_main / onc_metric_final kernels
+   (translated @ol.metric_rule methods) over the sim's committed cells and
+   writes the nvals reduced metric words (binary32 / int32 / uint32 bits, in
+   the metrics class's field order, nvals <= 64) to out (host memory). Sums
+   in a fixed order set by the cell count (ONC_METRIC_CPT in onc_device.h),
+   so binary32 metrics are reproducible; integer metrics wrap mod 2^32.
+   (replaces the atomicAdd metric kernels of _shader_builder.py:193-204 and
+   _stmt_translator.py:240-268, and the metric loop of CpuBackend.run_step,
+   cpu/_cpu_backend.py:65-70) */
+int onc_sim_metrics(onc_sim* sim, onc_module* module, int nvals, uint32_t* out);
 /* sum over completed steps of the live cells at the start of each step */
 int onc_sim_cell_updates(onc_sim* sim, uint64_t* cell_updates);
 /* device bytes this sim holds (HBM admission accounting) */

@@ -506,3 +506,23 @@ struct onc_rule_args {
     onc_u32 passes;
     const onc_u32 ONC_GLOBAL* cell_prefix;
 };
+
+// Arguments of the two kernels of a translated metric module (onc_sim_metrics).
+// onc_metric_main: block b takes the committed cells [b, b+1) x 256 x
+// ONC_METRIC_CPT; thread t adds the contributions of cells
+// (b ONC_METRIC_CPT + j) 256 + t for j = 0, 1, ... in that order, then the
+// block adds its 256 per-thread values in a fixed tree (stride 128, 64, ...,
+// 1) and writes one partial per metric. onc_metric_final: thread t adds
+// partials t, t + 256, ... in that order, then the same tree. The order is
+// fixed by the cell count alone, so binary32 metrics are reproducible.
+#define ONC_METRIC_CPT 8
+struct onc_metric_args {
+    const onc_dsim ONC_GLOBAL* S;    // the sim's segment-table entry
+    onc_u32 n, split, parity;        // committed cells: st[parity][0, split), then dau2[0, n - split)
+    onc_u32 ONC_GLOBAL* partials;    // [blocks][metrics]
+};
+struct onc_metric_fin_args {
+    const onc_u32 ONC_GLOBAL* partials;
+    onc_u32 nblocks;
+    onc_u32 ONC_GLOBAL* out;         // [metrics], binary32 / int32 / uint32 bits
+};

@@ -21,6 +21,7 @@
 #define ONC_DERR_SCAN 4u
 #define ONC_DERR_BIG_BIN 8u
 #define ONC_MAX_BIN_CELLS 65535u
+#define ONC_MAX_METRICS 64  // metric fields of one metric module
 #define ONC_SCAN_MIN_TILE 4096u  // fewest elements per tile of the exclusive scan (status words to allocate)
 
 // Width in float4 columns of a float4 stencil tile: the smallest power of two

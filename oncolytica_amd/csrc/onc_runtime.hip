@@ -567,6 +567,46 @@ extern "C" int onc_sim_download_cells(onc_sim* s, void* const* cols, uint32_t ca
     return ONC_OK;
 }
 
+static int ctx_scratch(onc_ctx* c, size_t bytes, void** out);
+
+extern "C" int onc_sim_metrics(onc_sim* s, onc_module* m, int nvals, uint32_t* out) {
+    if (!s || !m || !out) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (nvals < 1 || nvals > ONC_MAX_METRICS) return fail(ONC_ERR_INVALID, "nvals must be in [1, 64]");
+    CK(hipSetDevice(s->ctx->device));
+    hipFunction_t fm = nullptr, ff = nullptr;
+    if (hipModuleGetFunction(&fm, m->mod, "onc_metric_main") != hipSuccess ||
+        hipModuleGetFunction(&ff, m->mod, "onc_metric_final") != hipSuccess)
+        return fail(ONC_ERR_INVALID, "module has no metric kernels (onc_metric_main, onc_metric_final)");
+    onc_dstate st;
+    CKR(sim_read_state(s, &st));
+    onc_ctx* c = s->ctx;
+    const uint32_t per = 256u * ONC_METRIC_CPT;
+    const uint32_t nb = std::max<uint32_t>(1u, (uint32_t)(((uint64_t)st.n + per - 1) / per));
+    const size_t o_part = (sizeof(onc_dsim) + 255) & ~(size_t)255;
+    const size_t o_out = o_part + (((size_t)nb * nvals * 4 + 255) & ~(size_t)255);
+    void* base;
+    CKR(ctx_scratch(c, o_out + (size_t)nvals * 4, &base));
+    char* B = (char*)base;
+    CK(hipMemcpyAsync(B, &s->dsim, sizeof(onc_dsim), hipMemcpyHostToDevice, c->stream));
+    onc_metric_args a{};
+    a.S = (const onc_dsim*)B;
+    a.n = st.n;
+    a.split = st.split;
+    a.parity = st.parity;
+    a.partials = (onc_u32*)(B + o_part);
+    void* pa[] = {&a};
+    CK(hipModuleLaunchKernel(fm, nb, 1, 1, 256, 1, 1, 0, c->stream, pa, nullptr));
+    onc_metric_fin_args f{};
+    f.partials = a.partials;
+    f.nblocks = nb;
+    f.out = (onc_u32*)(B + o_out);
+    void* pf[] = {&f};
+    CK(hipModuleLaunchKernel(ff, 1, 1, 1, 256, 1, 1, 0, c->stream, pf, nullptr));
+    CK(hipMemcpyAsync(out, f.out, (size_t)nvals * 4, hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    return ONC_OK;
+}
+
 extern "C" int onc_sim_upload_field(onc_sim* s, int f, const float* host) {
     if (!s || !host) return fail(ONC_ERR_INVALID, "NULL argument");
     if (f < 0 || f >= s->desc.n_fields) return fail(ONC_ERR_INVALID, "no such field");

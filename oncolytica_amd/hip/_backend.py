@@ -11,8 +11,12 @@ sync_to_host() downloads SoA columns and fields (replaces the per-object
                unpack of _gpu_backend.py:373-405);
 sync_to_device() uploads them (replaces _gpu_backend.py:353-371).
 
-Metric rules run on the host over the synced cells when collect_metrics is
-set (the cell objects are rebuilt as the reference's sync_to_host does).
+Metric rules are translated with the cell rules into two more kernels of
+the same module; with collect_metrics set, run_step reduces them over the
+committed cells on the device (onc_sim_metrics: fixed summation order, so
+binary32 metrics are reproducible) and sets the Metrics fields from the
+result (replaces the atomicAdd metric kernels, _shader_builder.py:193-204,
+and the host loop of cpu/_cpu_backend.py:65-70).
 """
 from __future__ import annotations
 
@@ -71,6 +75,7 @@ class HipBackend:
         self.tissue: Optional[TissueSchema] = None
         self.source: str = ""
         self._next_id = 0
+        self.metric_fields: list = []  # (name, "f32" / "i32" / "u32") per metric word
 
     # ---- compile ---------------------------------------------------------------------
     def compile(self, sim_instance: Any) -> None:
@@ -84,7 +89,8 @@ class HipBackend:
         self.schema = CellSchema.of(spec.cell_class)
         self.tissue = TissueSchema.of(spec.tissue_class if e.tissue is not None else None)
         tr = Translator(sim_instance, self.schema, self.tissue, sim_instance._params)
-        self.source = tr.translate(e._cell_rules)
+        self.source = tr.translate(e._cell_rules, e._metric_rules, spec.metrics_class)
+        self.metric_fields = list(tr.metric_fields)
         self.module = Module(self.ctx, self.source)
         self.sim = Sim(self.ctx, self._desc(sim_instance))
         self.sim.set_rule(self.module, Translator.KERNEL)
@@ -135,12 +141,22 @@ class HipBackend:
     def run_step(self, collect_metrics: bool = False) -> None:
         self.sim.step(1)
         if collect_metrics and self.engine._metric_rules:
-            self.sync_to_host()
-            m = self.engine._metrics
-            m._clear()
-            for rule in self.engine._metric_rules:
-                for item in self.engine._resolve_container_for(rule):
-                    rule(item, m)
+            self.collect_metrics()
+
+    def collect_metrics(self) -> None:
+        """Reduce the metric rules over the committed cells on the device and
+        set the engine's Metrics fields."""
+        words = self.sim.metrics(self.module, len(self.metric_fields))
+        m = self.engine._metrics
+        m._clear()
+        for (name, t), w in zip(self.metric_fields, words):
+            if t == "f32":
+                v = float(np.array([w], dtype=np.uint32).view(np.float32)[0])
+            elif t == "i32":
+                v = int(np.array([w], dtype=np.uint32).view(np.int32)[0])
+            else:
+                v = int(w)
+            setattr(m, name, v)
 
     def run_steps(self, n: int) -> None:
         self.sim.step(n)

@@ -117,6 +117,13 @@ class Sim:
     def sync(self) -> None:
         call("onc_sim_sync", self.h)
 
+    def metrics(self, module: "Module", nvals: int) -> np.ndarray:
+        """The metric words (uint32 bits) of the module's metric kernels over
+        the committed cells."""
+        out = np.zeros(nvals, dtype=np.uint32)
+        call("onc_sim_metrics", self.h, module.h, nvals, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return out
+
     def close(self) -> None:
         if self.h:
             _abi.lib().onc_sim_destroy(self.h)

@@ -48,7 +48,7 @@ CTYPE = {F32: "float", I32: "int", U32: "unsigned", BOOL: "bool", VEC3: "onc_v3"
 
 @dataclass(frozen=True)
 class StructT:
-    kind: str   # "cell" | "tissue"
+    kind: str   # "cell" | "tissue" | "metrics"
     cls: type
 
     @property
@@ -334,9 +334,11 @@ class FuncInfo:
 
 class Translator:
     """Translates the cell rules of one Simulation subclass into a HIP
-    module with one kernel, `onc_rule_main`."""
+    module with one kernel, `onc_rule_main`, and its metric rules (if any)
+    into `onc_metric_main` + `onc_metric_final` (onc_sim_metrics)."""
 
     KERNEL = "onc_rule_main"
+    METRIC_KERNELS = ("onc_metric_main", "onc_metric_final")
 
     def __init__(self, sim, cell_schema: CellSchema, tissue: TissueSchema, params):
         self.sim = sim
@@ -352,9 +354,11 @@ class Translator:
         self.lines: list[str] = []
         self.indent = 0
         self.cur: Optional[FuncInfo] = None
+        self.in_metric = False
+        self.metric_fields: list[tuple[str, str]] = []  # (name, f32 / i32 / u32), in word order
 
     # ---- entry ---------------------------------------------------------------------
-    def translate(self, rules: list) -> str:
+    def translate(self, rules: list, metric_rules: list = (), metrics_cls=None) -> str:
         self.struct_of(StructT("cell", self.cell.cls))
         if self.tissue.cls is not None:
             self.struct_of(StructT("tissue", self.tissue.cls))
@@ -371,7 +375,33 @@ class Translator:
             self.funcs[key] = info
             self._emit_func(info, is_rule=True)
             rule_names.append(info.cname)
-        return self._assemble(rule_names)
+        metric_names = []
+        if metric_rules:
+            if metrics_cls is None:
+                raise CompilationError("metric rules need a Metrics class")
+            mt = self.struct_of(StructT("metrics", metrics_cls))
+            self.metric_fields = [(n, t) for n, t in self.struct_fields(mt)]
+            if not self.metric_fields:
+                raise CompilationError(f"'{metrics_cls.__name__}' has no metric fields")
+            if len(self.metric_fields) > 64:
+                raise CompilationError("at most 64 metric fields")
+        for r in metric_rules:
+            fn = r.__func__ if hasattr(r, "__func__") else r
+            node = self._parse(fn)
+            args = node.args.args
+            if len(args) != 3:
+                raise CompilationError(f"metric rule '{fn.__name__}' must take (self, cell, metrics)", node)
+            key = f"metric:{fn.__name__}"
+            info = FuncInfo(key, f"metric_{fn.__name__}", node, _namespace(fn), "sim", VOID,
+                            [(args[1].arg, StructT("cell", self.cell.cls)), (args[2].arg, mt)])
+            self.funcs[key] = info
+            self.in_metric = True
+            try:
+                self._emit_func(info, is_rule=True)
+            finally:
+                self.in_metric = False
+            metric_names.append(info.cname)
+        return self._assemble(rule_names, metric_names)
 
     def _parse(self, fn) -> ast.FunctionDef:
         try:
@@ -390,6 +420,17 @@ class Translator:
         return st
 
     def struct_fields(self, st: StructT) -> list:
+        if st.kind == "metrics":
+            out = []
+            for n, ann in data_fields(st.cls):
+                try:
+                    t = py_to_dsl(ann)
+                except TypeError:
+                    t = None
+                if t not in (F32, I32, U32):
+                    raise CompilationError(f"metric field '{n}' must be ol.f32, ol.i32 or ol.u32")
+                out.append((n, t))
+            return out
         if st.kind == "cell":
             return list(self.cell.fields) if st.cls is self.cell.cls else [(n, py_to_dsl(t)) for n, t in data_fields(st.cls)]
         return [(n, t) for n, t, _ in self.tissue.fields]
@@ -625,7 +666,28 @@ class Translator:
         c, t = self.expr(value, env)
         self.emit(f"{lc} = {self.conv(c, t, lt, node)};")
 
+    def _metric_target(self, node, env):
+        """(C expression, type) of `m.field` when node is a metric field."""
+        if not (isinstance(node, ast.Attribute) and isinstance(node.value, ast.Name)):
+            return None
+        loc = env.get(node.value.id)
+        if loc is None or not (isinstance(loc.t, StructT) and loc.t.kind == "metrics"):
+            return None
+        return f"{loc.cname}.{node.attr}", self.field_type(loc.t, node.attr, node)
+
     def st_AugAssign(self, node, env):
+        mtgt = self._metric_target(node.target, env)
+        if mtgt is not None:
+            if not isinstance(node.op, (ast.Add, ast.Sub)):
+                raise CompilationError("metric fields take += and -= only", node)
+            lc, lt = mtgt
+            sign = "+" if isinstance(node.op, ast.Add) else "-"
+            rc = self.conv(*self.expr(node.value, env), lt, node)
+            if lt == I32:  # wraps mod 2^32, as the reference's atomicAdd / atomicSub
+                self.emit(f"{lc} = (int)((unsigned){lc} {sign} (unsigned)({rc}));")
+            else:
+                self.emit(f"{lc} = {lc} {sign} {rc};")
+            return
         lc, lt = self.lvalue(node.target, env)
         rc, rt = self.binop(node.op, lc, lt, *self.expr(node.value, env), node)
         self.emit(f"{lc} = {self.conv(rc, rt, lt, node)};")
@@ -639,10 +701,14 @@ class Translator:
                 raise CompilationError("neighbour cells are read-only", node)
             return loc.cname, loc.t
         if isinstance(node, ast.Attribute):
+            if self._metric_target(node, env) is not None:
+                raise CompilationError("metric fields take += and -= only", node)
             bc, bt = self.expr(node.value, env)
             if isinstance(bt, StructT):
                 if bt.kind == "tissue":
                     raise CompilationError("cell rules may not write tissue fields", node)
+                if self.in_metric and bt.kind == "cell":
+                    raise CompilationError("metric rules read cells; they may not write them", node)
                 if bc.startswith("_nb"):
                     raise CompilationError("neighbour cells are read-only", node)
                 return f"{bc}.{node.attr}", self.field_type(bt, node.attr, node)
@@ -710,6 +776,9 @@ class Translator:
         raise CompilationError("only 'for x in range(...)' and 'for nb in <cell>.neighbors' loops are supported", node)
 
     def _for_neighbors(self, node, env):
+        if self.in_metric:
+            raise CompilationError("metric rules may not walk neighbours (the bins index the step's cells, "
+                                   "not the committed ones)", node)
         src = node.iter.value
         sc, st = self.expr(src, env)
         if not (isinstance(st, StructT) and st.kind == "cell"):
@@ -878,6 +947,8 @@ class Translator:
                 return self.literal(obj, node)
         bc, bt = self.expr(node.value, env)
         if isinstance(bt, StructT):
+            if bt.kind == "metrics":
+                raise CompilationError("metric fields can only be added to inside a metric rule", node)
             return f"{bc}.{node.attr}", self.field_type(bt, node.attr, node)
         if bt in (VEC3, IVEC3) and node.attr in ("x", "y", "z"):
             return f"{bc}.{node.attr}", F32 if bt == VEC3 else I32
@@ -1029,6 +1100,8 @@ class Translator:
             return self.conv(*args[0], U32, node), U32
         if name == "bool":
             return self.conv(*args[0], BOOL, node), BOOL
+        if name in ("random", "random_dir") and self.in_metric:
+            raise CompilationError("metric rules may not draw random numbers", node)
         if name == "random" and (obj is T.random or root is not None):
             return "onc_rand(ctx.rng)", F32
         if name == "random_dir":
@@ -1140,7 +1213,7 @@ class Translator:
                 lines.append(f"    r.{n} = {self.conv(c, lt, t)};")
         return lines
 
-    def _assemble(self, rule_names: list[str]) -> str:
+    def _assemble(self, rule_names: list[str], metric_names: list[str] = ()) -> str:
         cs = StructT("cell", self.cell.cls)
         out = [PRELUDE]
         out.append("struct RuleCtx;\n")
@@ -1220,4 +1293,76 @@ class Translator:
     }});
 }}
 """)
+        if metric_names:
+            out.append(self._metric_kernels(metric_names))
         return "\n".join(out)
+
+    def _metric_kernels(self, metric_names: list[str]) -> str:
+        """onc_metric_main / onc_metric_final (onc_sim_metrics): per-thread
+        sums in cell order, then fixed trees (the order in onc_device.h)."""
+        cs = StructT("cell", self.cell.cls)
+        mt = next(st for st in self.structs.values() if st.kind == "metrics")
+        nv = len(self.metric_fields)
+
+        def bits(t, v):
+            return f"__float_as_uint({v})" if t == F32 else f"(onc_u32)({v})"
+
+        def add(t, a, b):
+            if t == F32:
+                return f"__float_as_uint(__uint_as_float({a}) + __uint_as_float({b}))"
+            return f"{a} + {b}"
+
+        zero = "\n".join(f"    m.{n} = {self.zero(t)};" for n, t in self.metric_fields)
+        calls = "\n".join(f"        {r}(cell, m, ctx);" for r in metric_names)
+        tree = []
+        for k, (n, t) in enumerate(self.metric_fields):
+            tree.append(f"""    s_v[t] = {bits(t, f"m.{n}")};
+    __syncthreads();
+    for (onc_u32 h = 128u; h > 0u; h >>= 1) {{
+        if (t < h) s_v[t] = {add(t, "s_v[t]", "s_v[t + h]")};
+        __syncthreads();
+    }}
+    if (t == 0u) a.partials[(size_t)blockIdx.x * {nv}u + {k}u] = s_v[0];
+    __syncthreads();""")
+        fin = []
+        for k, (n, t) in enumerate(self.metric_fields):
+            fin.append(f"""    {{
+        onc_u32 v = 0u;
+        for (onc_u32 q = t; q < a.nblocks; q += 256u) v = {add(t, "v", f"a.partials[(size_t)q * {nv}u + {k}u]")};
+        s_v[t] = v;
+        __syncthreads();
+        for (onc_u32 h = 128u; h > 0u; h >>= 1) {{
+            if (t < h) s_v[t] = {add(t, "s_v[t]", "s_v[t + h]")};
+            __syncthreads();
+        }}
+        if (t == 0u) a.out[{k}u] = s_v[0];
+        __syncthreads();
+    }}""")
+        return (f"""extern "C" __global__ __launch_bounds__(256) void onc_metric_main(onc_metric_args a) {{
+    __shared__ onc_u32 s_v[256];
+    const onc_dsim& S = *a.S;
+    const onc_u32 t = threadIdx.x;
+    {mt.cname} m;
+{zero}
+    for (onc_u32 j = 0; j < ONC_METRIC_CPT; j++) {{
+        const onc_u32 i = (blockIdx.x * ONC_METRIC_CPT + j) * 256u + t;
+        if (i >= a.n) break;
+        RuleCtx ctx;
+        ctx.S = &S;
+        ctx.par = a.parity ^ 1u;
+        ctx.i = i;
+        ctx.epoch = 0u;
+        ctx.alive = true;
+        ctx.div = false;
+        {cs.cname} cell = i < a.split ? {cs.cname}_load(S, S.st[a.parity], i) : {cs.cname}_load(S, S.dau2, i - a.split);
+{calls}
+    }}
+{chr(10).join(tree)}
+}}
+
+extern "C" __global__ __launch_bounds__(256) void onc_metric_final(onc_metric_fin_args a) {{
+    __shared__ onc_u32 s_v[256];
+    const onc_u32 t = threadIdx.x;
+{chr(10).join(fin)}
+}}
+""")

@@ -27,6 +27,9 @@ class TumorCell(ol.Cell):
 
 class SpheroidMetrics(ol.Metrics):
     alive: ol.u32 = 0
+    age_sum: ol.f32 = 0.0      # / alive: mean age
+    oxygen_sum: ol.f32 = 0.0   # / alive: mean O2 at the cells
+    dividing: ol.u32 = 0       # old enough and oxygenated enough to divide next step
 
 
 class SpheroidParams(ol.Params):
@@ -60,4 +63,9 @@ class Spheroid(ol.Simulation[SpheroidTissue, SpheroidChem, TumorCell, SpheroidMe
 
     @ol.metric_rule
     def count(self, cell: TumorCell, m: SpheroidMetrics):
+        t: SpheroidTissue = self.tissue_at(cell.pos)
         m.alive += 1
+        m.age_sum += cell.age
+        m.oxygen_sum += t.oxygen
+        if cell.age + 1.0 > self.params.div_age and t.oxygen > self.params.div_o2:
+            m.dividing += 1

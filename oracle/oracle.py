@@ -107,6 +107,56 @@ def exscan(a) -> tuple[np.ndarray, int]:
     return out, int(tot)
 
 
+def metric_reduce(contrib, kind: str, cpt: int = 8) -> int:
+    """Sum of per-cell metric contributions in the order of the metric
+    kernels (onc_metric_main / onc_metric_final); returns the 32-bit word.
+
+    The reference adds metrics with atomicAdd / atomicSub
+    (gpu/compiler/translator/_stmt_translator.py:240-268; its f32 helper is
+    never defined) after clearing them (gpu/runtime/_gpu_backend.py:118-120,
+    157-166), and the CPU backend loops over the cells
+    (cpu/_cpu_backend.py:65-70). Integer sums wrap mod 2^32 in any order;
+    binary32 sums here follow the fixed order: block b holds cells
+    [b, b+1) x 256 cpt, thread t adds cells (b cpt + j) 256 + t for
+    j = 0..cpt-1 from +0.0, the block adds thread values in a tree (t += t + h
+    for h = 128, 64, ..., 1), and the last stage adds block partials
+    t, t + 256, ... per thread from +0.0, then the same tree."""
+    c = np.asarray(contrib)
+    if kind == "f32":
+        dt = np.float32
+        c = c.astype(np.float32)
+    elif kind in ("i32", "u32"):
+        dt = np.uint32
+        c = (c.astype(np.int64) & 0xFFFFFFFF).astype(np.uint32)
+    else:
+        raise ValueError(kind)
+    n = len(c)
+    per = 256 * cpt
+    nb = max(1, -(-n // per))
+    x = np.zeros(nb * per, dtype=dt)
+    x[:n] = c
+    x = x.reshape(nb, cpt, 256)
+    acc = np.zeros((nb, 256), dtype=dt)
+    with np.errstate(over="ignore"):
+        for j in range(cpt):
+            acc = acc + x[:, j, :]
+
+        def tree(v):
+            h = v.shape[-1] // 2
+            while h >= 1:
+                v[..., :h] = v[..., :h] + v[..., h:2 * h]
+                h //= 2
+            return v[..., 0]
+
+        part = tree(acc)
+        v = np.zeros(256, dtype=dt)
+        for q0 in range(0, nb, 256):
+            ch = part[q0:q0 + 256]
+            v[:len(ch)] = v[:len(ch)] + ch
+        r = tree(v)
+    return int(np.asarray(r, dtype=dt).view(np.uint32))
+
+
 def bins(sorted_keys, table: int, fill: int = 0xFFFFFFFF) -> tuple[np.ndarray, np.ndarray]:
     k = np.ascontiguousarray(sorted_keys, dtype=np.uint32)
     s = np.full(table, fill, dtype=np.uint32)

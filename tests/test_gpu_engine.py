@@ -112,6 +112,7 @@ def test_die_and_metrics():
     eng.load_model(DieSim())
     eng.run_step(collect_metrics=True)
     assert eng.get_metrics().alive_count == 1
+    eng.sync_to_host()  # metrics are reduced on the device; the cells stay there until synced
     assert [c.type_id for c in eng.cells._data] == [2]
     assert math.isclose(eng.cells._data[0].pos.x, 5.0)
     eng.run_step(collect_metrics=True)
@@ -309,3 +310,105 @@ def test_update_params_reaches_device():
     eng.run_step()
     eng.sync_to_host()
     assert eng.cells._data[0].v == 11.0
+
+
+def test_metric_rules_reduce_on_device_like_the_oracle():
+    """@ol.metric_rule on the device (onc_sim_metrics): u32 and i32 sums are
+    exact (mod 2^32), the binary32 sum is bit-identical to the oracle's fixed
+    summation order over the committed cells, and a metric that reads the
+    swept tissue counts the same cells as numpy does."""
+    class MCell(ol.Cell):
+        pos: ol.vec3
+        energy: ol.f32 = 1.0
+        kind: ol.i32 = 0
+
+    class MTissue(ol.Tissue):
+        oxygen: ol.f32 = ol.diffusing(rate=0.1, init=1.0, boundary=ol.dirichlet(0.2))
+
+    class MMetrics(ol.Metrics):
+        alive: ol.u32 = 0
+        energy_sum: ol.f32 = 0.0
+        net: ol.i32 = 0
+        rich: ol.u32 = 0
+
+    class MSim(ol.Simulation[MTissue, MyChem, MCell, MMetrics, MyParams]):
+        @ol.cell_rule
+        def burn(self, cell: MCell):
+            cell.energy = cell.energy * 0.75 + 0.001 * ol.f32(cell.kind)
+            if cell.kind == 6 and cell.energy < 0.3:
+                cell.die()
+
+        @ol.metric_rule
+        def count(self, cell: MCell, m: MMetrics):
+            m.alive += 1
+            m.energy_sum += cell.energy
+            if cell.kind > 2:
+                m.net -= 1
+            else:
+                m.net += cell.kind
+
+        @ol.metric_rule
+        def oxygenated(self, cell: MCell, m: MMetrics):
+            t: MTissue = self.tissue_at(cell.pos)
+            if t.oxygen > 0.6:
+                m.rich += 1
+
+    n = 20_000
+    rng = np.random.default_rng(11)
+    pos = rng.random((n, 3)) * 60.0 + 2.0
+    energy = rng.random(n).astype(np.float32) * 3.0
+    kind = rng.integers(0, 8, n)
+    eng = ol.Engine(backend="hip", max_agents=n)
+    eng.setup_geometry((64, 64, 64), 1.0)
+    eng.setup_tissue(MTissue)
+    for i in range(n):
+        eng.cells.add(MCell(pos=ol.vec3(*pos[i]), energy=float(energy[i]), kind=int(kind[i])))
+    eng.load_model(MSim())
+    for step in range(4):
+        eng.run_step(collect_metrics=True)
+        m = eng.get_metrics()
+        eng.sync_to_host()
+        cells = eng.cells._data
+        e = np.array([c.energy for c in cells], dtype=np.float32)
+        k = np.array([c.kind for c in cells], dtype=np.int64)
+        p = np.array([[c.pos.x, c.pos.y, c.pos.z] for c in cells], dtype=np.float32)
+        ox = eng._backend_impl.sim.download_field(0)
+        v = np.clip(np.floor(p).astype(np.int64), 0, 63)
+        o = ox[v[:, 0] + 64 * (v[:, 1] + 64 * v[:, 2])]
+        print(f"step {step}: {len(cells)} cells, metrics {m.alive} {m.energy_sum} {m.net} {m.rich}", flush=True)
+        assert m.alive == len(cells)
+        want = O.metric_reduce(e, "f32")
+        got = int(np.array([m.energy_sum], dtype=np.float32).view(np.uint32)[0])
+        assert got == want, (m.energy_sum, np.uint32(want).view(np.float32))
+        assert m.net == int(np.where(k > 2, -1, k).sum())
+        assert m.rich == int((o > np.float32(0.6)).sum())
+    assert len(cells) < n  # some cells died
+
+
+def test_spheroid_metrics_match_the_committed_cells():
+    """The benchmark model's metric rule (population, summed age, summed O2
+    at the cells, cells ready to divide) after 30 steps of the 4,096-cell
+    workload: the integer metrics equal numpy's counts and the binary32 sums
+    the oracle's fixed order, over the downloaded cells and field."""
+    w = W.plain_c1()
+    cols = W.initial_cells(w)
+    eng = ol.Engine(backend="hip", max_agents=1 << 16, seed=w.seed)
+    eng.setup_geometry((64, 64, 64), 1.0, cell_diameter=1.0)
+    eng.setup_mechanics(k_rep=1.0, k_adh=0.0, mobility=0.25, max_disp=0.25, interaction_radius=1.0,
+                        cell_radius=0.5)
+    eng.setup_tissue(SpheroidTissue)
+    eng.cells.set_columns({"pos": np.stack(cols[:3], 1), "_id": cols[3], "age": cols[4], "generation": cols[5]})
+    eng.load_model(Spheroid())
+    eng.run_steps(29)
+    eng.run_step(collect_metrics=True)
+    m = eng.get_metrics()
+    eng.sync_to_host()
+    got = eng.cells.columns()
+    age = got["age"].astype(np.float32)
+    p = got["pos"].astype(np.float32)
+    v = np.clip(np.floor(p).astype(np.int64), 0, 63)
+    o2 = eng.tissue.flat("oxygen")[v[:, 0] + 64 * (v[:, 1] + 64 * v[:, 2])]
+    assert m.alive == len(age) > 4096
+    assert np.float32(m.age_sum).view(np.uint32) == O.metric_reduce(age, "f32")
+    assert np.float32(m.oxygen_sum).view(np.uint32) == O.metric_reduce(o2, "f32")
+    assert m.dividing == int(((age + np.float32(1.0) > np.float32(24.0)) & (o2 > np.float32(0.3))).sum())

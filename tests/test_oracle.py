@@ -376,3 +376,64 @@ def test_step_identical_for_any_thread_count():
         O.set_threads(n_threads)
     for a, b in zip(out[0][0] + out[0][1], out[1][0] + out[1][1]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+# ---- metric reductions ----------------------------------------------------------------
+def _metric_reduce_scalar(contrib, cpt=8):
+    """The same order as O.metric_reduce, one binary32 addition at a time."""
+    f = np.float32
+    n = len(contrib)
+    per = 256 * cpt
+    nb = max(1, -(-n // per))
+    part = []
+    for b in range(nb):
+        th = []
+        for t in range(256):
+            a = f(0.0)
+            for j in range(cpt):
+                i = (b * cpt + j) * 256 + t
+                if i < n:
+                    a = f(a + f(contrib[i]))
+            th.append(a)
+        h = 128
+        while h >= 1:
+            for t in range(h):
+                th[t] = f(th[t] + th[t + h])
+            h //= 2
+        part.append(th[0])
+    v = [f(0.0)] * 256
+    for q in range(nb):
+        v[q % 256] = f(v[q % 256] + part[q])
+    h = 128
+    while h >= 1:
+        for t in range(h):
+            v[t] = f(v[t] + v[t + h])
+        h //= 2
+    return int(np.array([v[0]], dtype=np.float32).view(np.uint32)[0])
+
+
+def test_metric_reduce_f32_matches_scalar_order():
+    rng = np.random.default_rng(3)
+    for n in (0, 1, 255, 2048, 2049, 5000):
+        c = (rng.standard_normal(n) * 10).astype(np.float32)
+        assert O.metric_reduce(c, "f32") == _metric_reduce_scalar(c), n
+
+
+def test_metric_reduce_exact_cases():
+    # sums that binary32 holds exactly give the exact value in any order
+    for n in (0, 1, 3000, 70_000):
+        assert np.uint32(O.metric_reduce(np.ones(n, np.float32), "f32")).view(np.float32) == n
+    c = np.arange(10_000) % 7
+    assert np.uint32(O.metric_reduce(c.astype(np.float32), "f32")).view(np.float32) == c.sum()
+    # integers wrap mod 2^32 (atomicAdd / atomicSub semantics)
+    rng = np.random.default_rng(4)
+    c = rng.integers(-2**31, 2**31, 100_000)
+    assert O.metric_reduce(c, "i32") == int(c.sum()) & 0xFFFFFFFF
+    assert O.metric_reduce(np.full(600_000, 10_000), "u32") == (6_000_000_000 & 0xFFFFFFFF)
+    # the order in closed form: 2^24 then 4095 ones. Thread 0 of block 0
+    # holds cells 0, 256, ..., 1792 and absorbs its seven ones into 2^24 (a
+    # plain sequential sum absorbs all of them); every other thread sums its
+    # eight ones exactly, so the result is 2^24 + 4088.
+    c = np.ones(4096, dtype=np.float32)
+    c[0] = 2.0 ** 24
+    assert np.uint32(O.metric_reduce(c, "f32")).view(np.float32) == 2.0 ** 24 + 4088

@@ -189,3 +189,98 @@ def test_chemistry_access_rejected():
 
     with pytest.raises(ol.CompilationError, match="chemistry"):
         _bad(uses_chem)()
+
+
+class MMetrics(ol.Metrics):
+    alive: ol.u32 = 0
+    energy_sum: ol.f32 = 0.0
+    net: ol.i32 = 0
+
+
+class MSim(ol.Simulation[BTissue, BChem, BCell, MMetrics, BParams]):
+    @ol.cell_rule
+    def move(self, cell: BCell):
+        cell.energy -= 0.01
+
+    @ol.metric_rule
+    def count(self, cell: BCell, m: MMetrics):
+        m.alive += 1
+        m.energy_sum += cell.energy
+        if cell.kind > 2:
+            m.net -= 1
+        else:
+            m.net += cell.kind
+
+    @ol.metric_rule
+    def rich(self, cell: BCell, m: MMetrics):
+        t: BTissue = self.tissue_at(cell.pos)
+        if t.oxygen > self.params.speed:
+            m.net += 2
+
+
+def translate_metrics(sim, metric_rules):
+    tr = Translator(sim, CellSchema.of(BCell), TissueSchema.of(BTissue), sim._params)
+    src = tr.translate([sim.move], metric_rules, MMetrics)
+    return tr, src
+
+
+def test_metric_rules_translate_and_compile():
+    """@ol.metric_rule methods become onc_metric_main (per-thread sums in cell
+    order, a fixed tree per block) and onc_metric_final; the word order is the
+    Metrics class's field order."""
+    sim = MSim()
+    tr, src = translate_metrics(sim, [sim.count, sim.rich])
+    assert tr.metric_fields == [("alive", "u32"), ("energy_sum", "f32"), ("net", "i32")]
+    for frag in ("onc_metric_main", "onc_metric_final", "metric_count(cell, m, ctx);", "metric_rich(cell, m, ctx);",
+                 "(int)((unsigned)m_.net - (unsigned)(1))", "m_.energy_sum = m_.energy_sum + "):
+        assert frag in src, frag
+    _compile(src)
+
+
+def test_metric_rule_restrictions():
+    def reads(self, cell: BCell, m: MMetrics):
+        if m.alive > 3:
+            m.alive += 1
+
+    def assigns(self, cell: BCell, m: MMetrics):
+        m.alive = 1
+
+    def multiplies(self, cell: BCell, m: MMetrics):
+        m.energy_sum *= 2.0
+
+    def draws(self, cell: BCell, m: MMetrics):
+        m.energy_sum += ol.random()
+
+    def walks(self, cell: BCell, m: MMetrics):
+        for nb in cell.neighbors:
+            m.alive += 1
+
+    def writes_cell(self, cell: BCell, m: MMetrics):
+        cell.energy = 0.0
+        m.alive += 1
+
+    def kills(self, cell: BCell, m: MMetrics):
+        cell.die()
+
+    for f in (reads, assigns, multiplies, draws, walks, writes_cell, kills):
+        class S(MSim):
+            extra = ol.metric_rule(f)
+
+        sim = S()
+        with pytest.raises(ol.CompilationError):
+            translate_metrics(sim, [sim.extra])
+
+
+def test_metric_fields_must_be_scalars():
+    class VMetrics(ol.Metrics):
+        centre: ol.vec3
+
+    class VSim(ol.Simulation[BTissue, BChem, BCell, VMetrics, BParams]):
+        @ol.metric_rule
+        def c(self, cell: BCell, m: VMetrics):
+            pass
+
+    sim = VSim()
+    tr = Translator(sim, CellSchema.of(BCell), TissueSchema.of(BTissue), sim._params)
+    with pytest.raises(ol.CompilationError, match="metric field"):
+        tr.translate([], [sim.c], VMetrics)
