@@ -223,6 +223,44 @@ def metrics_leg(ctx, desc, cols0, Sim):
     return out
 
 
+def bundle_leg(ctx, desc, cols0, Sim):
+    """Checkpoint of the 1M-cell sim after 10 steps (oncolytica_amd.bundle):
+    export to a temporary directory and restore on the same device, on the
+    host clock; the restored sim's next step is compared bit for bit with
+    the original's."""
+    import shutil
+    import tempfile
+    import numpy as np
+    from oncolytica_amd import bundle as B
+    from oncolytica_amd import workloads as W
+    sim = Sim(ctx, desc)
+    sim.upload_cells(cols0)
+    sim.step(10)
+    sim.sync()
+    d = tempfile.mkdtemp(prefix="onc_bundle_")
+    try:
+        t0 = time.perf_counter()
+        B.export_bundle(sim, d, W.TUMOR_COL_NAMES)
+        t_exp = time.perf_counter() - t0
+        nbytes = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+        t0 = time.perf_counter()
+        r, _, cfg = B.load_bundle(ctx, d)
+        t_load = time.perf_counter() - t0
+        sim.step(1)
+        r.step(1)
+        same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+                   for a, b in zip(sim.download_cells(), r.download_cells()))
+        r.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+        sim.close()
+    out = {"cells": cfg["state"]["n"], "bytes": nbytes, "export_s": t_exp, "restore_s": t_load,
+           "next_step_bit_identical": bool(same)}
+    _log(f"bundle: {out['cells']:,} cells, {nbytes / 1e6:.0f} MB, export {t_exp:.2f} s, restore {t_load:.2f} s, "
+         f"next step bit-identical {same}")
+    return out
+
+
 def ensemble(ctx, n_sims: int, steps: int):
     """Batched uneven ensemble (64 sims, cells log-uniform in [10k, 500k],
     grids from {64, 128, 192}^3, per-sim uptake / division age / hypoxia):
@@ -772,6 +810,7 @@ def main():
             "events_every_steps": args.event_every, "launches_timed": dom_n}
 
     met = metrics_leg(ctx, desc, cols0, Sim)
+    bun = bundle_leg(ctx, desc, cols0, Sim)
     kern = None if args.no_kernels else kernel_rooflines(ctx)
     cpu = None if args.no_cpu_baseline else cpu_baseline(w, args.cpu_steps, args.cpu_steps_1t)
     ens = ensemble(ctx, args.ensemble_sims, args.ensemble_steps) if args.ensemble_sims > 0 else None
@@ -802,6 +841,7 @@ def main():
         "phases_separate_launches": table_sep,
         "kernels": kern,
         "metrics": met,
+        "bundle": bun,
         "value_separate_launches": sep_value,
         "cpu_baseline": cpu,
         "ensemble": ens,

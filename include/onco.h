@@ -173,6 +173,11 @@ int onc_sim_counts(onc_sim* sim, uint32_t* n, uint32_t* next_id, uint32_t* steps
    batch hold jobs with different horizons without host round trips).
    Default: no limit (0xFFFFFFFF). */
 int onc_sim_set_stop(onc_sim* sim, uint32_t stop);
+/* sets the completed-step counter (the Philox counter word of the next
+   step's draws): restores a sim from a bundle so that it continues with the
+   draws it would have made (no counterpart in the reference, which has no
+   loader; its export writes state.bin, _gpu_backend.py:295-334) */
+int onc_sim_set_step(onc_sim* sim, uint32_t step);
 /* metric rules: runs the module's onc_metric_main / onc_metric_final kernels
    (translated @ol.metric_rule methods) over the sim's committed cells and
    writes the nvals reduced metric words (binary32 / int32 / uint32 bits, in

@@ -637,6 +637,16 @@ extern "C" int onc_sim_set_stop(onc_sim* s, uint32_t stop) {
     return ONC_OK;
 }
 
+extern "C" int onc_sim_set_step(onc_sim* s, uint32_t step) {
+    if (!s) return fail(ONC_ERR_INVALID, "NULL sim");
+    CK(hipSetDevice(s->ctx->device));
+    onc_dstate st;
+    CKR(sim_read_state(s, &st));
+    st.step = step;
+    CK(hipMemcpy(s->dstate, &st, sizeof(st), hipMemcpyHostToDevice));
+    return ONC_OK;
+}
+
 extern "C" int onc_sim_counts(onc_sim* s, uint32_t* n, uint32_t* next_id, uint32_t* steps) {
     if (!s) return fail(ONC_ERR_INVALID, "NULL sim");
     CK(hipSetDevice(s->ctx->device));

@@ -106,6 +106,7 @@ _PROTOS = {
     "onc_sim_sync": (C.c_int, [_P]),
     "onc_sim_counts": (C.c_int, [_P, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
     "onc_sim_set_stop": (C.c_int, [_P, C.c_uint32]),
+    "onc_sim_set_step": (C.c_int, [_P, C.c_uint32]),
     "onc_sim_metrics": (C.c_int, [_P, _P, C.c_int, C.POINTER(_u32)]),
     "onc_sim_footprint": (C.c_int, [C.POINTER(SimDesc), C.POINTER(C.c_uint64)]),
     "onc_sim_cell_updates": (C.c_int, [_P, C.POINTER(C.c_uint64)]),

@@ -21,6 +21,8 @@ and the host loop of cpu/_cpu_backend.py:65-70).
 from __future__ import annotations
 
 import atexit
+import json
+import os
 import typing
 from typing import Any, Optional
 
@@ -160,6 +162,40 @@ class HipBackend:
 
     def run_steps(self, n: int) -> None:
         self.sim.step(n)
+
+    # ---- bundle (checkpoint) ---------------------------------------------------------
+    def get_config(self) -> dict:
+        """The bundle config of the compiled sim's committed state
+        (replaces GpuBackend.get_config, _gpu_backend.py:196-293)."""
+        if self.sim is None:
+            raise RuntimeError("Simulation must be compiled before generating config.")
+        from .. import bundle
+        return bundle.get_config(self.sim, [c.name for c in self.schema.columns], Translator.KERNEL,
+                                 self.metric_fields)
+
+    def export_bundle(self, export_dir: str, export_initial_state: bool = True) -> str:
+        """Writes config.json and rule.hip, and with export_initial_state the
+        cells and fields as the device holds them now (replaces
+        GpuBackend.export_bundle, _gpu_backend.py:295-334, which re-uploads
+        the host objects first; here the device state is the current one).
+        Returns the path of config.json; oncolytica_amd.bundle.load_bundle
+        restores it."""
+        if self.sim is None:
+            raise RuntimeError("Simulation must be compiled before exporting a bundle.")
+        from .. import bundle
+        names = [c.name for c in self.schema.columns]
+        if export_initial_state:
+            return bundle.export_bundle(self.sim, export_dir, names, self.source, Translator.KERNEL,
+                                        self.metric_fields)
+        os.makedirs(export_dir, exist_ok=True)
+        cfg = self.get_config()
+        cfg["buffers"] = {}
+        with open(os.path.join(export_dir, "rule.hip"), "w", encoding="utf-8") as f:
+            f.write(self.source)
+        path = os.path.join(export_dir, "config.json")
+        with open(path, "w", encoding="utf-8") as f:
+            json.dump(cfg, f, indent=1)
+        return path
 
     def get_metrics(self) -> Any:
         return self.engine._metrics

@@ -117,6 +117,10 @@ class Sim:
     def sync(self) -> None:
         call("onc_sim_sync", self.h)
 
+    def set_step(self, step: int) -> None:
+        """Set the completed-step counter (restoring from a bundle)."""
+        call("onc_sim_set_step", self.h, int(step))
+
     def metrics(self, module: "Module", nvals: int) -> np.ndarray:
         """The metric words (uint32 bits) of the module's metric kernels over
         the committed cells."""

@@ -70,3 +70,18 @@ def test_footprint_and_validation_without_device():
     bad.dims[0] = 2000
     with pytest.raises(_abi.OncError):
         _abi.footprint(bad)
+
+
+def test_bundle_descriptor_round_trip_and_format_check(tmp_path):
+    """A sim descriptor survives config.json exactly (binary32 constants
+    through JSON doubles), and a directory with another format is refused
+    before any device work."""
+    import json
+    from oncolytica_amd import bundle as B
+    from oncolytica_amd import workloads as W
+    d = W.large_c2(n_cells=1000).desc()
+    d2 = B.desc_from_dict(json.loads(json.dumps(B.desc_to_dict(d))))
+    assert bytes(d) == bytes(d2)
+    (tmp_path / "config.json").write_text(json.dumps({"format": "something-else", "version": 1}))
+    with pytest.raises(ValueError, match="bundle"):
+        B.load_bundle(None, str(tmp_path))
