@@ -18,16 +18,26 @@ runs and join the batch when it ends.
 
 A job's result does not depend on what else is in the batch: a batched step
 is bit-identical to stepping the sim alone.
+
+Preemption (optional, `evict_dir` and `preempt_after`): when the oldest
+waiting job has waited longer than `preempt_after` seconds and does not fit,
+the resident jobs with the most steps left are evicted between ticks, each
+to a bundle (oncolytica_amd.bundle: committed cells, fields, step counter),
+until it fits; they wait again in arrival order and are restored from their
+bundles when admitted, continuing bit for bit.
 """
 from __future__ import annotations
 
 import math
+import os
+import shutil
 import time
 from dataclasses import dataclass, field
 from typing import Callable, Optional
 
 import numpy as np
 
+from . import bundle as B
 from . import workloads as W
 from .hip import _abi
 from .hip._sim import Batch, Sim
@@ -47,6 +57,8 @@ class Job:
     finished: float = math.nan
     cell_updates: int = 0
     result: Optional[tuple] = None
+    bundle: Optional[str] = None   # where the job waits while evicted
+    evictions: int = 0
 
     @property
     def latency(self) -> float:
@@ -64,6 +76,7 @@ class ServeReport:
     peak_bytes: int
     budget_bytes: int
     host_times: dict = field(default_factory=dict)
+    evictions: int = 0
 
     def summary(self) -> dict:
         lat = np.array([j.latency for j in self.jobs if not math.isnan(j.finished)])
@@ -82,6 +95,7 @@ class ServeReport:
             "peak_resident_jobs": self.peak_resident,
             "peak_resident_bytes": self.peak_bytes,
             "hbm_budget_bytes": self.budget_bytes,
+            "evictions": self.evictions,
             "host_times_s": self.host_times,
         }
 
@@ -112,7 +126,8 @@ def poisson_jobs(rate: float, duration: float, seed: int = 42, scale: float = 1.
 class Server:
     def __init__(self, ctx: _abi.Context, hbm_budget: int, max_jobs: int = 256, max_tick_steps: int = 64,
                  graph_min_steps: int = 4, keep_results: bool = False, tuning=None,
-                 log: Optional[Callable[[str], None]] = None):
+                 log: Optional[Callable[[str], None]] = None, evict_dir: Optional[str] = None,
+                 preempt_after: float = math.inf):
         self.ctx = ctx
         self.budget = int(hbm_budget)
         self.max_jobs = max_jobs
@@ -121,6 +136,10 @@ class Server:
         self.keep_results = keep_results
         self.tuning = tuning
         self.log = log
+        self.evict_dir = evict_dir
+        self.preempt_after = preempt_after
+        if evict_dir is None and math.isfinite(preempt_after):
+            raise ValueError("preemption needs evict_dir")
 
     @staticmethod
     def _prepare(job: Job) -> None:
@@ -145,7 +164,7 @@ class Server:
         staged: list[Job] = []     # admitted: resident on the device, joining the batch at the next tick
         active: list[Job] = []     # in the batch
         used = 0
-        ticks = graph_ticks = peak_res = peak_bytes = 0
+        ticks = graph_ticks = peak_res = peak_bytes = evictions = 0
         total_cu = 0
         times = {"admit_s": 0.0, "retire_s": 0.0, "launch_s": 0.0, "gpu_wait_s": 0.0}
         t0 = clock()
@@ -164,13 +183,20 @@ class Server:
                     break
                 if used + j.footprint > self.budget:
                     continue
-                self._prepare(j)
-                j.sim = Sim(self.ctx, j.workload.desc())
-                j.sim.upload_cells(j.cols0)
-                j.sim.set_stop(j.horizon)
-                j.cols0 = None
-                j.left = j.horizon
-                j.admitted = clock() - t0
+                if j.bundle is not None:
+                    # an evicted job: restored with its cells, fields and step counter
+                    j.sim, _, _ = B.load_bundle(self.ctx, j.bundle)
+                    j.sim.set_stop(j.horizon)
+                    shutil.rmtree(j.bundle, ignore_errors=True)
+                    j.bundle = None
+                else:
+                    self._prepare(j)
+                    j.sim = Sim(self.ctx, j.workload.desc())
+                    j.sim.upload_cells(j.cols0)
+                    j.sim.set_stop(j.horizon)
+                    j.cols0 = None
+                    j.left = j.horizon
+                    j.admitted = clock() - t0
                 used += j.footprint
                 waiting.remove(j)
                 staged.append(j)
@@ -216,8 +242,9 @@ class Server:
                     j.left = max(0, j.left - k)
                     if j.left == 0:
                         j.finished = done_t
-                        j.cell_updates = _cu(j.sim)
-                        total_cu += j.cell_updates
+                        cu = _cu(j.sim)
+                        j.cell_updates += cu
+                        total_cu += cu
                         if self.keep_results:
                             j.result = (j.sim.download_cells(),
                                         [j.sim.download_field(f) for f in range(j.sim.desc.n_fields)])
@@ -227,15 +254,44 @@ class Server:
                         used -= j.footprint
                         active.remove(j)
                 times["retire_s"] += clock() - tr
+                # preemption: the oldest job that has not run yet has waited
+                # too long and does not fit; evict the jobs with the most steps
+                # left (evicted jobs never preempt: no ping-pong)
+                fresh = [w for w in waiting if math.isnan(w.admitted)]
+                if fresh and self.evict_dir is not None:
+                    head = min(fresh, key=lambda w: w.arrival)
+                    if (clock() - t0) - head.arrival > self.preempt_after and used + head.footprint > self.budget:
+                        for v in sorted(active, key=lambda a: (-a.left, a.id)):
+                            if used + head.footprint <= self.budget:
+                                break
+                            v.evictions += 1
+                            evictions += 1
+                            v.bundle = os.path.join(self.evict_dir, f"job{v.id}_{v.evictions}")
+                            B.export_bundle(v.sim, v.bundle)
+                            cu = _cu(v.sim)
+                            v.cell_updates += cu
+                            total_cu += cu
+                            batch.remove(v.sim)
+                            v.sim.close()
+                            v.sim = None
+                            used -= v.footprint
+                            active.remove(v)
+                            waiting.append(v)
+                            waiting.sort(key=lambda w: w.arrival)
+                            if self.log:
+                                self.log(f"t={clock() - t0:7.2f}s evicted job {v.id} ({v.left} steps left)")
         finally:
             for j in active + staged:
                 if j.sim is not None:
                     j.sim.close()
+            for j in jobs:
+                if j.bundle is not None:
+                    shutil.rmtree(j.bundle, ignore_errors=True)
             batch.close()
         wall = clock() - t0
         return ServeReport(jobs=jobs, wall_s=wall, cell_updates=total_cu, ticks=ticks, graph_ticks=graph_ticks,
                            peak_resident=peak_res, peak_bytes=peak_bytes, budget_bytes=self.budget,
-                           host_times=times)
+                           host_times=times, evictions=evictions)
 
 
 def _cu(sim: Sim) -> int:

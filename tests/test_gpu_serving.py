@@ -220,3 +220,29 @@ def test_profile_every_samples_steps(ctx):
         b.close()
         for s in sims:
             s.close()
+
+
+def test_preempted_jobs_resume_bit_exact(ctx, tmp_path):
+    """With room for one job at a time and preemption after 0 s, every newly
+    arrived job evicts the running one to a bundle; evicted jobs are restored
+    later and every job's result equals the job stepped alone."""
+    jobs = _jobs(3, seed=5)
+    for j, h in zip(jobs, (40, 30, 20)):
+        j.arrival = 0.0
+        j.horizon = h
+    budget = max(footprint(j.workload.desc()) for j in jobs)
+    rep = Server(ctx, hbm_budget=budget, max_tick_steps=8, keep_results=True, evict_dir=str(tmp_path),
+                 preempt_after=0.0).run(jobs)
+    s = rep.summary()
+    assert s["completed"] == 3 and s["evictions"] >= 2 and rep.peak_resident == 1
+    assert sum(j.evictions for j in jobs) == s["evictions"]
+    for j in jobs:
+        cols, fields = run_alone(ctx, j)
+        _same(cols, fields, *j.result)
+    assert not any(p.name.startswith("job") for p in tmp_path.iterdir())  # bundles removed on restore
+    assert rep.cell_updates == sum(j.cell_updates for j in jobs) > 0
+
+
+def test_preemption_needs_a_directory(ctx):
+    with pytest.raises(ValueError, match="evict_dir"):
+        Server(ctx, hbm_budget=1 << 30, preempt_after=1.0)
