@@ -115,7 +115,8 @@ def kernel_rooflines(ctx):
     (u32 key, u32 value) pairs at 24 and 27 key bits (3 and 4 passes of
     16 B/pair; the 256^3 and 512^3 key widths), exclusive scan of 64M u32
     (8 B/element), one 7-point sweep of a 512^3 binary32 field (8 B/voxel),
-    Morton keys of 16M positions (12 B read + 4 B written per cell)."""
+    Morton keys of 16M positions (12 B read + 4 B written per cell), raw
+    neighbour forces of 1M key-sorted cells (28 B per cell; latency-bound)."""
     import numpy as np
     from oncolytica_amd.hip import _abi
     from oncolytica_amd.hip._abi import BC_DIRICHLET, FieldDesc
@@ -175,6 +176,32 @@ def kernel_rooflines(ctx):
           lambda: _abi.call("onc_morton_keys", ctx.h, px.p, py.p, pz.p, n, dims.ctypes.data_as(_abi._i32p), 1.0,
                             _abi.KEY_MORTON32, keys.p), n=n)
     for q in (px, py, pz, keys):
+        q.free()
+    # raw neighbour forces (onc_forces) of the 1M initial cells of the
+    # 256^3 workload, sorted by key with the pure Morton and sort kernels;
+    # 28 B per cell (packed position read, force written): the walk is bound
+    # by the latency of its gathers, not by these bytes
+    from oncolytica_amd import workloads as W
+    w2 = W.large_c2(n_cells=1_000_000)
+    d2 = w2.desc()
+    c2 = W.initial_cells(w2)
+    n = len(c2[0])
+    dims2 = np.array(list(d2.dims), dtype=np.int32)
+    src = [ctx.upload(np.ascontiguousarray(c2[k], dtype=np.float32)) for k in range(3)]
+    keys = ctx.alloc(4 * n)
+    _abi.call("onc_morton_keys", ctx.h, src[0].p, src[1].p, src[2].p, n, dims2.ctypes.data_as(_abi._i32p), d2.dx,
+              d2.key_format, keys.p)
+    idx = ctx.upload(np.arange(n, dtype=np.uint32))
+    ko, vo = ctx.alloc(4 * n), ctx.alloc(4 * n)
+    _abi.call("onc_sort_pairs", ctx.h, keys.p, idx.p, n, 24, ko.p, vo.p)
+    perm = vo.download(np.uint32, n)
+    ps = [ctx.upload(np.ascontiguousarray(c2[k][perm], dtype=np.float32)) for k in range(3)]
+    fo = [ctx.alloc(4 * n) for _ in range(3)]
+    timed("forces_1M", 28.0 * n,
+          lambda: _abi.call("onc_forces", ctx.h, d2, ps[0].p, ps[1].p, ps[2].p, n, fo[0].p, fo[1].p, fo[2].p),
+          n=n, cells_per_s=None)
+    out["forces_1M"]["cells_per_s"] = n / (out["forces_1M"]["us"] * 1e-6)
+    for q in src + ps + fo + [keys, idx, ko, vo]:
         q.free()
     ctx.timing(False)
     return out
@@ -490,6 +517,7 @@ PURE_ALG_BYTES = {
     "k_exscan": (None, "8 B per element"),
     "k_stencil": (8 * 512 ** 3, "512^3 binary32 read + written"),
     "k_morton_keys": (16 * (16 << 20), "16M positions (12 B) read, keys (4 B) written"),
+    "k_forces_raw": (28 * 1_000_000, "1M packed positions (16 B) read, forces (12 B) written"),
 }
 
 

@@ -243,6 +243,15 @@ int onc_morton_keys(onc_ctx* ctx, const float* px, const float* py, const float*
  * (replaces the atomic counting sort of template.wgsl:165-186, 335-343) */
 int onc_sort_pairs(onc_ctx* ctx, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t n,
                    int key_bits, uint32_t* keys_out, uint32_t* vals_out);
+/* raw neighbour forces of n cells whose positions (device pointers) are
+ * sorted by Morton key (stable, as after a step's sort): for each cell the
+ * linear repulsion / adhesion taper summed over the cells of its 27 Moore
+ * voxels in visiting order, before integration; the sim descriptor gives the
+ * grid, key format and force constants (its fields and capacity are not
+ * used). (replaces the neighbour loop of the cell-rule kernel,
+ * _stmt_translator.py:459-491, with the force law of tumor_sim.wgsl:505-628) */
+int onc_forces(onc_ctx* ctx, const onc_sim_desc* desc, const float* px, const float* py, const float* pz,
+               uint32_t n, float* fx, float* fy, float* fz);
 /* exclusive prefix sum; *total (device pointer, may be NULL) receives the sum
  * (replaces template.wgsl:192-329, Blelloch 3-level scan) */
 int onc_exscan_u32(onc_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total);

@@ -59,6 +59,12 @@ hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s);
 hipError_t onc_launch_morton(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_hist_only(const onc_kargs& a, int items, int grid, hipStream_t s);
 hipError_t onc_launch_sort_pass(const onc_kargs& a, int items, int pass, int grid, hipStream_t s);
+// onc_forces: pack positions and keys, bin entries (epoch 1), then the raw
+// force walk of one sim table entry S (device) with a.cell_prefix = {0, n};
+// t0 / t1 (may be null) bracket the force kernel alone
+hipError_t onc_launch_forces_raw(const onc_kargs& a, const float* px, const float* py, const float* pz, onc_u32 n,
+                                 const onc_dsim* S, float* pos4, onc_u32* keys, onc_u32* bins, onc_u32* err,
+                                 float* fx, float* fy, float* fz, hipEvent_t t0, hipEvent_t t1, hipStream_t s);
 // the three-launch radix pass of large standalone sorts (tiles of 256 x ONC_SORT3_ITEMS keys)
 #define ONC_SORT3_ITEMS 16
 #define ONC_SORT3_MIN_N (1u << 20)

@@ -847,8 +847,11 @@ struct onc_forces_lds {
 
 // Forces of cell i of sim s through global memory (bins table, packed
 // positions). Must be called by every lane of the wave that has a cell (the
-// list flush is wave-wide).
-__device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u32 i, onc_forces_lds& L) {
+// list flush is wave-wide). RAW: write the summed force to (rx, ry, rz)[i]
+// instead of integrating the position (the onc_forces entry point).
+template <bool RAW = false>
+__device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u32 i, onc_forces_lds& L,
+                                                float* rx = nullptr, float* ry = nullptr, float* rz = nullptr) {
     onc_u32 (*s_pairs)[ONC_FORCES_BLOCK] = L.pairs;
     onc_u32 (*s_rs)[ONC_FORCES_BLOCK] = L.rs;
     unsigned short (*s_rl)[ONC_FORCES_BLOCK] = L.rl;
@@ -919,6 +922,12 @@ __device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u
                     if (__ballot(npair > ONC_PAIR_LIST / 2) != 0ull) flush();
                 });
     flush();
+    if (RAW) {
+        rx[i] = Fx;
+        ry[i] = Fy;
+        rz[i] = Fz;
+        return;
+    }
     const float mob = S.mobility, md = S.max_disp;
     float dpx = mob * Fx, dpy = mob * Fy, dpz = mob * Fz;
     float l2 = dpx * dpx + dpy * dpy;
@@ -952,6 +961,44 @@ __device__ __forceinline__ void onc_forces_wave(const onc_kargs& a, onc_u32 blk,
 __global__ __launch_bounds__(ONC_FORCES_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_forces(onc_kargs a) {
     __shared__ onc_forces_lds L;
     onc_forces_wave(a, blockIdx.x, gridDim.x, L);
+}
+
+// The summed forces alone (onc_forces): the same walk and pair law, written
+// out instead of integrated.
+__global__ __launch_bounds__(ONC_FORCES_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void k_forces_raw(onc_kargs a, float* fx, float* fy, float* fz) {
+    __shared__ onc_forces_lds L;
+    onc_for_chunks_xcd(a.cell_prefix, a.nsims, blockIdx.x, gridDim.x,
+                       [&](int s, onc_u32 i) __attribute__((always_inline)) {
+                           onc_forces_cell<true>(a, s, i, L, fx, fy, fz);
+                       });
+}
+
+// Packed positions and keys of caller-supplied cells (onc_forces).
+__global__ __launch_bounds__(256) void k_pack_keys(const float* __restrict__ px, const float* __restrict__ py,
+                                                   const float* __restrict__ pz, onc_u32 n, const onc_dsim* S,
+                                                   float4* __restrict__ pos4, onc_u32* __restrict__ keys) {
+    const onc_dsim& D = *S;
+    for (onc_u32 i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const float x = px[i], y = py[i], z = pz[i];
+        pos4[i] = make_float4(x, y, z, 0.0f);
+        keys[i] = onc_cell_key(D, x, y, z);
+    }
+}
+
+// Bin entries {start, end, epoch, 0} of sorted keys: the first cell of every
+// run of equal keys counts the run (bins longer than the force walk's limit
+// set *err).
+__global__ __launch_bounds__(256) void k_bins4(const onc_u32* __restrict__ keys, onc_u32 n, uint4* bins,
+                                               onc_u32 epoch, onc_u32* err) {
+    for (onc_u32 i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const onc_u32 k = keys[i];
+        if (i > 0u && keys[i - 1u] == k) continue;
+        onc_u32 e = i + 1u;
+        while (e < n && keys[e] == k) e++;
+        bins[k] = make_uint4(i, e, epoch, 0u);
+        if (e - i > ONC_MAX_BIN_CELLS) atomicOr(err, ONC_DERR_BIG_BIN);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1738,6 +1785,19 @@ hipError_t onc_launch_sort_down(const onc_u32* kin, const onc_u32* vin, onc_u32*
                                 onc_u32 shift, const onc_u32* offsets, onc_u32 T, hipStream_t s) {
     hipLaunchKernelGGL(k_sort_down<ONC_SORT3_ITEMS>, dim3(T), dim3(256), 0, s, kin, vin, kout, vout, n, shift,
                        offsets, T);
+    return hipGetLastError();
+}
+hipError_t onc_launch_forces_raw(const onc_kargs& a, const float* px, const float* py, const float* pz, onc_u32 n,
+                                 const onc_dsim* S, float* pos4, onc_u32* keys, onc_u32* bins, onc_u32* err,
+                                 float* fx, float* fy, float* fz, hipEvent_t t0, hipEvent_t t1, hipStream_t s) {
+    const unsigned g = (unsigned)std::min<size_t>((n + 255u) / 256u, 65535u);
+    hipLaunchKernelGGL(k_pack_keys, dim3(g), dim3(256), 0, s, px, py, pz, n, S, (float4*)pos4, keys);
+    hipLaunchKernelGGL(k_bins4, dim3(g), dim3(256), 0, s, keys, n, (uint4*)bins, 1u, err);
+    // one wave per 64 cells, a multiple of 8 blocks (onc_for_chunks_xcd)
+    const unsigned waves = (unsigned)((((size_t)n + 63u) / 64u + 7u) / 8u * 8u);
+    if (t0) (void)hipEventRecord(t0, s);
+    hipLaunchKernelGGL(k_forces_raw, dim3(waves ? waves : 8u), dim3(ONC_FORCES_BLOCK), 0, s, a, fx, fy, fz);
+    if (t1) (void)hipEventRecord(t1, s);
     return hipGetLastError();
 }
 hipError_t onc_launch_exscan(const onc_u32* in, onc_u32* out, onc_u32 n, onc_u32* total, onc_u32* status,

@@ -1232,6 +1232,63 @@ extern "C" int onc_morton_keys(onc_ctx* c, const float* px, const float* py, con
     return tread(c);
 }
 
+extern "C" int onc_forces(onc_ctx* c, const onc_sim_desc* d, const float* px, const float* py, const float* pz,
+                          uint32_t n, float* fx, float* fy, float* fz) {
+    if (!c || !d || (n && (!px || !py || !pz || !fx || !fy || !fz))) return fail(ONC_ERR_INVALID, "NULL argument");
+    CKR(validate_desc(d));
+    if (n > (1u << 30)) return fail(ONC_ERR_INVALID, "n > 2^30");
+    CK(hipSetDevice(c->device));
+    if (n == 0) return ONC_OK;
+    uint32_t table = 0;
+    int key_bits = 0;
+    size_t nvox = 0;
+    uint64_t bytes = 0;
+    sim_layout(d, &table, &key_bits, &nvox, &bytes);
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        size_t o = off;
+        off += (b + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_sim = take(sizeof(onc_dsim)), o_small = take(256), o_pos = take((size_t)n * 16),
+                 o_keys = take((size_t)n * 4), o_bins = take((size_t)table * 16);
+    void* base;
+    CKR(ctx_scratch(c, off, &base));
+    char* B = (char*)base;
+    onc_dsim ds{};
+    for (int k = 0; k < 3; k++) ds.dims[k] = d->dims[k];
+    ds.dx = d->dx;
+    ds.key_format = d->key_format;
+    ds.R = d->interaction_radius;
+    ds.cell_radius = d->cell_radius;
+    ds.k_rep = d->k_rep;
+    ds.k_adh = d->k_adh;
+    ds.mobility = d->mobility;
+    ds.max_disp = d->max_disp;
+    ds.step = 0;  // bin entries carry epoch step + 1 = 1
+    ds.n_live = n;
+    ds.pos4 = (float*)(B + o_pos);
+    ds.bins = (onc_u32*)(B + o_bins);
+    onc_u32 small[4] = {0u, n, 0u, 0u};  // cell prefix {0, n}, error word
+    hipStream_t st = c->stream;
+    CK(hipMemcpyAsync(B + o_sim, &ds, sizeof(ds), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(B + o_small, small, sizeof(small), hipMemcpyHostToDevice, st));
+    CK(hipMemsetAsync(B + o_bins, 0, (size_t)table * 16, st));
+    onc_kargs a{};
+    a.sims = (const onc_dsim*)(B + o_sim);
+    a.nsims = 1;
+    a.cell_prefix = (onc_u32*)(B + o_small);
+    onc_u32* err = (onc_u32*)(B + o_small) + 2;
+    CK(onc_launch_forces_raw(a, px, py, pz, n, (const onc_dsim*)(B + o_sim), (float*)(B + o_pos),
+                             (onc_u32*)(B + o_keys), (onc_u32*)(B + o_bins), err, fx, fy, fz,
+                             c->timing ? c->t0 : nullptr, c->timing ? c->t1 : nullptr, st));
+    onc_u32 herr = 0;
+    CK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (herr) return fail(ONC_ERR_DEVICE, "a voxel held more than " + std::to_string(ONC_MAX_BIN_CELLS) + " cells");
+    return tread(c);
+}
+
 // Large standalone sorts: each pass as counts -> scan of the counts -> rank
 // and scatter (three launches, no look-back chain between tiles).
 static int sort_pairs_3launch(onc_ctx* c, const uint32_t* kin, const uint32_t* vin, uint32_t n, int passes,

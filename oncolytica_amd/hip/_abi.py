@@ -126,6 +126,7 @@ _PROTOS = {
     "onc_batch_phase_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_int]),
     "onc_morton_keys": (C.c_int, [_P, _P, _P, _P, _u32, _i32p, C.c_float, C.c_int, _P]),
     "onc_sort_pairs": (C.c_int, [_P, _P, _P, _u32, C.c_int, _P, _P]),
+    "onc_forces": (C.c_int, [_P, C.POINTER(SimDesc), _P, _P, _P, _u32, _P, _P, _P]),
     "onc_exscan_u32": (C.c_int, [_P, _P, _P, _u32, _P]),
     "onc_bins": (C.c_int, [_P, _P, _u32, _P, _P]),
     "onc_stencil7": (C.c_int, [_P, _P, _P, _i32p, C.POINTER(FieldDesc)]),

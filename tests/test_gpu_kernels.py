@@ -217,3 +217,44 @@ def test_copy_u32(ctx, n, shift):
     got = dst.download(np.uint32, n + 8)
     np.testing.assert_array_equal(got[:n], a[shift:])
     assert (got[n:] == 0xDEADBEEF).all()
+
+
+def _forces_case(ctx, d, p):
+    k = O.keys(p[:, 0], p[:, 1], p[:, 2], tuple(d.dims), d.dx, d.key_format)
+    p = np.ascontiguousarray(p[np.argsort(k, kind="stable")])
+    n = len(p)
+    bufs = [ctx.upload(np.ascontiguousarray(p[:, a])) for a in range(3)]
+    outs = [ctx.alloc(4 * max(1, n)) for _ in range(3)]
+    _abi.call("onc_forces", ctx.h, d, *[b.p for b in bufs], n, *[o.p for o in outs])
+    got = [o.download(np.float32, n) for o in outs]
+    ref = O.forces_raw(d, p[:, 0], p[:, 1], p[:, 2], "f32")
+    for a in range(3):
+        np.testing.assert_array_equal(got[a].view(np.uint32), ref[a].view(np.uint32))
+    return got
+
+
+@pytest.mark.parametrize("case", ["c1_sphere", "dense_adhesion", "single"])
+def test_forces_bit_exact_with_oracle(ctx, case):
+    """onc_forces (the force walk and pair law alone, on key-sorted cells)
+    gives the oracle's binary32 forces bit for bit."""
+    from oncolytica_amd import workloads as Wl
+    if case == "c1_sphere":
+        w = Wl.plain_c1()
+        d = w.desc()
+        cols = Wl.initial_cells(w)
+        p = np.stack(cols[:3], 1).astype(np.float32)
+    elif case == "dense_adhesion":
+        w = Wl.plain_c1()
+        d = w.desc()
+        d.k_adh, d.interaction_radius = 0.2, 1.0
+        rng = np.random.default_rng(12)
+        p = (rng.random((30_000, 3)) * 12.0 + 26.0).astype(np.float32)  # ~17 cells per voxel
+    else:
+        w = Wl.plain_c1()
+        d = w.desc()
+        p = np.array([[10.5, 20.25, 30.0]], dtype=np.float32)
+    got = _forces_case(ctx, d, p)
+    if case == "single":
+        assert all(g[0] == 0.0 for g in got)
+    else:
+        assert max(np.abs(g).max() for g in got) > 0.0
