@@ -137,6 +137,11 @@ int onc_ctx_info(onc_ctx* ctx, onc_device_info* out);
 /* raw device memory, for the pure-kernel entry points below */
 int onc_dev_alloc(onc_ctx* ctx, size_t bytes, void** out);
 int onc_dev_free(onc_ctx* ctx, void* p);
+/* pinned (page-locked) host memory for large cell and field transfers
+   (hipHostMalloc; the reference's buffers are mapped by wgpu,
+   _device_manager.py:76-119) */
+int onc_host_alloc(onc_ctx* ctx, size_t bytes, void** out);
+int onc_host_free(onc_ctx* ctx, void* p);
 int onc_h2d(onc_ctx* ctx, void* dst, const void* src, size_t bytes);
 int onc_d2h(onc_ctx* ctx, void* dst, const void* src, size_t bytes);
 int onc_ctx_sync(onc_ctx* ctx);

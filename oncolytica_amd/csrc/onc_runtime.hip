@@ -221,6 +221,21 @@ extern "C" int onc_dev_free(onc_ctx* c, void* p) {
     CK(hipFree(p));
     return ONC_OK;
 }
+extern "C" int onc_host_alloc(onc_ctx* c, size_t bytes, void** out) {
+    if (!c || !out) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(c->device));
+    void* p = nullptr;
+    hipError_t e = hipHostMalloc(&p, bytes ? bytes : 4, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(ONC_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    *out = p;
+    return ONC_OK;
+}
+extern "C" int onc_host_free(onc_ctx* c, void* p) {
+    if (!c) return fail(ONC_ERR_INVALID, "NULL ctx");
+    CK(hipSetDevice(c->device));
+    if (p) CK(hipHostFree(p));
+    return ONC_OK;
+}
 extern "C" int onc_h2d(onc_ctx* c, void* dst, const void* src, size_t bytes) {
     if (!c) return fail(ONC_ERR_INVALID, "NULL ctx");
     if (!bytes) return ONC_OK;

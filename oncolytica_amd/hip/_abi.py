@@ -88,6 +88,8 @@ _PROTOS = {
     "onc_ctx_info": (C.c_int, [_P, C.POINTER(DeviceInfo)]),
     "onc_dev_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "onc_dev_free": (C.c_int, [_P, _P]),
+    "onc_host_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "onc_host_free": (C.c_int, [_P, _P]),
     "onc_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "onc_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "onc_ctx_sync": (C.c_int, [_P]),
@@ -216,6 +218,32 @@ class Context:
         v = C.c_double()
         call("onc_ctx_last_kernel_us", self.h, C.byref(v))
         return v.value
+
+
+class PinnedBuffer:
+    """Page-locked host memory (onc_host_alloc) viewed as a numpy array."""
+
+    def __init__(self, ctx: Context, count: int, dtype=np.uint32):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        p = _P()
+        call("onc_host_alloc", ctx.h, max(4, self.count * self.dtype.itemsize), C.byref(p))
+        self.p = p
+        buf = (C.c_char * max(4, self.count * self.dtype.itemsize)).from_address(p.value)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=self.count)
+
+    def free(self) -> None:
+        if self.p:
+            self.array = None
+            lib().onc_host_free(self.ctx.h, self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:

@@ -258,3 +258,18 @@ def test_forces_bit_exact_with_oracle(ctx, case):
         assert all(g[0] == 0.0 for g in got)
     else:
         assert max(np.abs(g).max() for g in got) > 0.0
+
+
+def test_pinned_host_buffers_round_trip(ctx):
+    """onc_host_alloc: page-locked host memory moves data to and from the
+    device like ordinary host memory."""
+    n = 3_000_001
+    src = _abi.PinnedBuffer(ctx, n, np.uint32)
+    dst = _abi.PinnedBuffer(ctx, n, np.uint32)
+    src.array[:] = np.arange(n, dtype=np.uint32) * 2654435761
+    dev = ctx.alloc(4 * n)
+    _abi.call("onc_h2d", ctx.h, dev.p, src.p, 4 * n)
+    _abi.call("onc_d2h", ctx.h, dst.p, dev.p, 4 * n)
+    np.testing.assert_array_equal(dst.array, src.array)
+    src.free()
+    dst.free()
