@@ -646,8 +646,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cells", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=20, help="oracle steps in the multi-core CPU baseline sample")
-    ap.add_argument("--cpu-steps-1t", type=int, default=3, help="oracle steps in the one-thread CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=80,
+                    help="oracle steps in the multi-core CPU baseline sample (~10 s on 16 threads)")
+    ap.add_argument("--cpu-steps-1t", type=int, default=10, help="oracle steps in the one-thread CPU sample (~10 s)")
     ap.add_argument("--stress", action="store_true",
                     help="also run the 16M-cell 512^3 stress workload (300 steps, mitotic burst) with its "
                          "full-size oracle parity and binary64 drift checkpoints")
