@@ -178,30 +178,34 @@ def kernel_rooflines(ctx):
     for q in (px, py, pz, keys):
         q.free()
     # raw neighbour forces (onc_forces) of the 1M initial cells of the
-    # 256^3 workload, sorted by key with the pure Morton and sort kernels;
-    # 28 B per cell (packed position read, force written): the walk is bound
-    # by the latency of its gathers, not by these bytes
+    # 256^3 workload, sorted on the host by their voxels' Morton codes (the
+    # standard 3-D interleave of 10-bit coordinates, which the 256^3 keys
+    # are); 28 B per cell (packed position read, force written): the walk is
+    # bound by the latency of its gathers, not by these bytes
     from oncolytica_amd import workloads as W
     w2 = W.large_c2(n_cells=1_000_000)
     d2 = w2.desc()
     c2 = W.initial_cells(w2)
     n = len(c2[0])
-    dims2 = np.array(list(d2.dims), dtype=np.int32)
-    src = [ctx.upload(np.ascontiguousarray(c2[k], dtype=np.float32)) for k in range(3)]
-    keys = ctx.alloc(4 * n)
-    _abi.call("onc_morton_keys", ctx.h, src[0].p, src[1].p, src[2].p, n, dims2.ctypes.data_as(_abi._i32p), d2.dx,
-              d2.key_format, keys.p)
-    idx = ctx.upload(np.arange(n, dtype=np.uint32))
-    ko, vo = ctx.alloc(4 * n), ctx.alloc(4 * n)
-    _abi.call("onc_sort_pairs", ctx.h, keys.p, idx.p, n, 24, ko.p, vo.p)
-    perm = vo.download(np.uint32, n)
-    ps = [ctx.upload(np.ascontiguousarray(c2[k][perm], dtype=np.float32)) for k in range(3)]
+
+    def spread10(a):
+        a = a.astype(np.uint32) & np.uint32(0x3FF)
+        a = (a | (a << np.uint32(16))) & np.uint32(0x030000FF)
+        a = (a | (a << np.uint32(8))) & np.uint32(0x0300F00F)
+        a = (a | (a << np.uint32(4))) & np.uint32(0x030C30C3)
+        return (a | (a << np.uint32(2))) & np.uint32(0x09249249)
+
+    vox = [np.clip(np.floor(np.asarray(c2[k], dtype=np.float32) / np.float32(d2.dx)), 0, d2.dims[k] - 1)
+           for k in range(3)]
+    order = np.argsort(spread10(vox[0]) | (spread10(vox[1]) << np.uint32(1)) | (spread10(vox[2]) << np.uint32(2)),
+                       kind="stable")
+    ps = [ctx.upload(np.ascontiguousarray(c2[k][order], dtype=np.float32)) for k in range(3)]
     fo = [ctx.alloc(4 * n) for _ in range(3)]
     timed("forces_1M", 28.0 * n,
           lambda: _abi.call("onc_forces", ctx.h, d2, ps[0].p, ps[1].p, ps[2].p, n, fo[0].p, fo[1].p, fo[2].p),
           n=n, cells_per_s=None)
     out["forces_1M"]["cells_per_s"] = n / (out["forces_1M"]["us"] * 1e-6)
-    for q in src + ps + fo + [keys, idx, ko, vo]:
+    for q in ps + fo:
         q.free()
     ctx.timing(False)
     return out
