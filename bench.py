@@ -675,8 +675,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(sims and ensembles shard by job across GPUs; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sims and ensembles would shard by job, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from oncolytica_amd.hip import _abi

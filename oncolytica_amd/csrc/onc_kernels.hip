@@ -1228,7 +1228,7 @@ __global__ __launch_bounds__(ONC_FORCES_BLOCK) __attribute__((amdgpu_waves_per_e
 
 
 // ---------------------------------------------------------------------------
-// Built-in tumour rule (the BASELINE model). Reads the permuted state (other
+// Built-in tumour rule (the benchmark workloads' model). Reads the permuted state (other
 // buffer), the integrated position and the swept fields; writes the cell's
 // columns to the current buffer, its alive/divide flags and its daughter.
 // ---------------------------------------------------------------------------

@@ -220,7 +220,7 @@ void FN(oracle_stencil7)(const REAL* in, REAL* out, const int32_t d[3], const on
 
 /* ---- neighbour forces -----------------------------------------------------
  * Pair law for delta = p_i - p_j, d = |delta| (tumor_sim.wgsl:541-570, with
- * the BASELINE linear spring in place of the Hertz k o sqrt(o), and a linear
+ * the benchmark model's linear spring in place of the Hertz k o sqrt(o), and a linear
  * taper in place of the cosine taper):
  *   d < S = 2 r_cell:      f = k_rep (S - d)            (repulsion)
  *   S <= d < R:            f = -k_adh (R - d) / (R - S) (adhesion)
@@ -312,8 +312,8 @@ static void integrate(const onc_sim_desc* d, REAL Fx, REAL Fy, REAL Fz, REAL* x,
     *z = clampr(*z + dpz, (REAL)0, (REAL)d->dims[2] * dx);
 }
 
-/* ---- built-in tumour rule (BASELINE model; the DSL form is
- * oncolytica_amd/models/spheroid.py) ----------------------------------------
+/* ---- built-in tumour rule (the benchmark model; the DSL form is
+ * oncolytica_amd/spheroid.py) ----------------------------------------
  * age += 1; die if O2 < hypoxia; else (two fields) die if rand() <
  * drug_kill * drug; else divide if age > div_age and O2 > div_o2: daughter
  * at p' + offset * random_dir(), age 0, generation + 1; parent age 0,
