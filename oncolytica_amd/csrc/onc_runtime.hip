@@ -884,7 +884,8 @@ static int grid_ctiles(onc_batch* b) { return (int)std::max<uint64_t>(1, b->ka.m
 // to global memory once per sim it touches; a block walks its share of the
 // cells in serial rounds of dependent loads, so more blocks (shorter shares)
 // finish sooner: four per CU instead of one made Morton 43 % faster on the
-// 64-sim ensemble, and the extra histogram atomics are spread over 8 copies.
+// 64-sim ensemble (eight were no faster), and the extra histogram atomics are
+// spread over 8 copies.
 static int grid_morton(onc_batch* b) {
     uint64_t need = (b->total_cap + 255) / 256;
     uint64_t lim = (uint64_t)b->ctx->cus * 4;
