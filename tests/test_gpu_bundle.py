@@ -77,3 +77,31 @@ def test_translated_rule_round_trip(ctx, tmp_path):
     finally:
         r.close()
         mod.close()
+
+
+def test_empty_sim_metrics_and_bundle(ctx, tmp_path):
+    """A sim with no cells: every metric is zero, and its bundle restores to
+    an empty sim that keeps its step counter and fields."""
+    from oncolytica_amd.hip._sim import Module
+    from oncolytica_amd.hip._translator import CellSchema, TissueSchema, Translator
+    from oncolytica_amd.spheroid import SpheroidMetrics, TumorCell
+    w = W.plain_c1()
+    sim = Sim(ctx, w.desc())
+    model = Spheroid()
+    tr = Translator(model, CellSchema.of(TumorCell), TissueSchema.of(SpheroidTissue), model._params)
+    mod = Module(ctx, tr.translate([model.lifecycle], [model.count], SpheroidMetrics))
+    try:
+        sim.upload_cells([c[:0] for c in W.initial_cells(w)])
+        sim.step(3)
+        sim.sync()
+        assert sim.counts()[0] == 0
+        np.testing.assert_array_equal(sim.metrics(mod, len(tr.metric_fields)), np.zeros(4, np.uint32))
+        r, _, cfg = B.load_bundle(ctx, B.export_bundle(sim, str(tmp_path / "e"), W.TUMOR_COL_NAMES))
+        try:
+            assert cfg["state"] == {"n": 0, "next_id": sim.counts()[1], "step": 3}
+            _same(sim, r, "empty")
+        finally:
+            r.close()
+    finally:
+        mod.close()
+        sim.close()
