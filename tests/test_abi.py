@@ -85,3 +85,18 @@ def test_bundle_descriptor_round_trip_and_format_check(tmp_path):
     (tmp_path / "config.json").write_text(json.dumps({"format": "something-else", "version": 1}))
     with pytest.raises(ValueError, match="bundle"):
         B.load_bundle(None, str(tmp_path))
+
+
+def test_new_entry_points_reject_null_handles():
+    """The metric, bundle, forces, timing-sample and pinned-memory entry
+    points refuse NULL handles with ONC_ERR_INVALID before any device work."""
+    L = _abi.lib()
+    out = (C.c_uint32 * 4)()
+    p = C.c_void_p()
+    assert L.onc_sim_metrics(None, None, 4, out) == _abi.ERR_INVALID
+    assert L.onc_sim_set_step(None, 3) == _abi.ERR_INVALID
+    assert L.onc_forces(None, None, None, None, None, 0, None, None, None) == _abi.ERR_INVALID
+    assert L.onc_batch_profile_every(None, 1, 2) == _abi.ERR_INVALID
+    assert L.onc_host_alloc(None, 64, C.byref(p)) == _abi.ERR_INVALID
+    assert L.onc_host_free(None, None) == _abi.ERR_INVALID
+    assert b"NULL" in L.onc_last_error()
