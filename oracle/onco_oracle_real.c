@@ -42,6 +42,16 @@
 #define RSQRT sqrt
 #define RFLOOR floor
 #endif
+/* Planted mistakes, for test builds only: tests/test_oracle_mutants.py builds
+ * this file with ORACLE_PLANT = 1..7 and checks that the closed-form step
+ * tests fail on each. The default build (0) plants nothing.
+ *   1 uptake run length off by one     2 uptake applied after the sweep
+ *   3 no max_disp clamp                4 no domain clamp
+ *   5 drug kill with <= instead of <   6 drug kill reads the second word
+ *   7 drug kill draws with step + 1 */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
 #define CAT2(a, b) a##_##b
 #define CAT(a, b) CAT2(a, b)
 #define FN(name) CAT(name, SFX)
@@ -300,13 +310,19 @@ static void integrate(const onc_sim_desc* d, REAL Fx, REAL Fy, REAL Fz, REAL* x,
     REAL dpx = mob * Fx, dpy = mob * Fy, dpz = mob * Fz;
     REAL l2 = dpx * dpx + dpy * dpy;
     l2 = l2 + dpz * dpz;
-    if (l2 > md * md) {
+    if (l2 > md * md && ORACLE_PLANT != 3) {
         REAL sc = md / RSQRT(l2);
         dpx = dpx * sc;
         dpy = dpy * sc;
         dpz = dpz * sc;
     }
     REAL dx = (REAL)d->dx;
+    if (ORACLE_PLANT == 4) {
+        *x = *x + dpx;
+        *y = *y + dpy;
+        *z = *z + dpz;
+        return;
+    }
     *x = clampr(*x + dpx, (REAL)0, (REAL)d->dims[0] * dx);
     *y = clampr(*y + dpy, (REAL)0, (REAL)d->dims[1] * dx);
     *z = clampr(*z + dpz, (REAL)0, (REAL)d->dims[2] * dx);
@@ -337,7 +353,7 @@ static void tumor_rule(const onc_sim_desc* d, uint32_t step, REAL* px, REAL* py,
     size_t v = (size_t)vox1(*px, dx, d->dims[0]) + X * (size_t)vox1(*py, dx, d->dims[1]) +
                XY * (size_t)vox1(*pz, dx, d->dims[2]);
     rng_t r;
-    r.id = id; r.step = step; r.c = 0;
+    r.id = id; r.step = ORACLE_PLANT == 7 ? step + 1u : step; r.c = 0;
     r.key[0] = (uint32_t)d->seed; r.key[1] = (uint32_t)(d->seed >> 32);
     REAL o2 = fields[0][v];
     *age = *age + (REAL)1;
@@ -349,8 +365,10 @@ static void tumor_rule(const onc_sim_desc* d, uint32_t step, REAL* px, REAL* py,
     }
     if (d->n_fields > 1 && P[ONC_TP_DRUG_KILL] > (REAL)0) {
         REAL drug = fields[1][v];
+        if (ORACLE_PLANT == 6) (void)rng_uniform(&r);
         REAL u = rng_uniform(&r);
-        if (u < P[ONC_TP_DRUG_KILL] * drug) {
+        if (ORACLE_PLANT == 7) r.step = step;
+        if (ORACLE_PLANT == 5 ? u <= P[ONC_TP_DRUG_KILL] * drug : u < P[ONC_TP_DRUG_KILL] * drug) {
             o->alive = 0;
             return;
         }
@@ -367,6 +385,32 @@ static void tumor_rule(const onc_sim_desc* d, uint32_t step, REAL* px, REAL* py,
         *age = (REAL)0;
         *gen = *gen + 1u;
         o->div = 1;
+    }
+}
+
+/* Uptake (tumor_sim.wgsl:387-412, made linear): each occupied voxel that is
+ * not fixed loses (cells in it) * uptake per field, clamped below at lo. The
+ * cells are key-sorted, so a voxel's cells form one run of equal keys; t[0..2]
+ * are their positions. Pinned by the closed-form step tests (a voxel holding
+ * k cells, the lo clamp, a fixed voxel, and the sweep seeing the uptake). */
+static void uptake(const onc_sim_desc* d, const uint32_t* keys, uint32_t n, REAL* const* t,
+                   REAL* const* fields) {
+    REAL dx = (REAL)d->dx;
+    size_t X = (size_t)d->dims[0], XY = (size_t)d->dims[0] * (size_t)d->dims[1];
+    for (uint32_t i = 0; i < n;) {
+        uint32_t e = i + 1;
+        while (e < n && keys[e] == keys[i]) e++;
+        int32_t vx = vox1(t[0][i], dx, d->dims[0]), vy = vox1(t[1][i], dx, d->dims[1]),
+                vz = vox1(t[2][i], dx, d->dims[2]);
+        size_t v = (size_t)vx + X * (size_t)vy + XY * (size_t)vz;
+        uint32_t run = ORACLE_PLANT == 1 ? e - i + 1u : e - i;
+        for (int f = 0; f < d->n_fields; f++) {
+            const onc_field_desc* fd = &d->fields[f];
+            if (fd->uptake == 0.0f || is_fixed(fd, d->dims, vx, vy, vz)) continue;
+            REAL u = fields[f][v] - (REAL)run * (REAL)fd->uptake;
+            fields[f][v] = u < (REAL)fd->lo ? (REAL)fd->lo : u;
+        }
+        i = e;
     }
 }
 
@@ -389,7 +433,6 @@ int FN(oracle_tumor_run)(const onc_sim_desc* d, uint32_t* n_io, uint32_t* next_i
     REAL* swept[ONC_MAX_FIELDS];
     for (int f = 0; f < d->n_fields; f++) swept[f] = (REAL*)malloc(nvox * sizeof(REAL));
     REAL dx = (REAL)d->dx;
-    size_t X = (size_t)d->dims[0], XY = (size_t)d->dims[0] * (size_t)d->dims[1];
 
     for (int s = 0; s < nsteps; s++) {
         uint32_t n = *n_io, step = *step_io;
@@ -404,20 +447,7 @@ int FN(oracle_tumor_run)(const onc_sim_desc* d, uint32_t* n_io, uint32_t* next_i
             tid[i] = id[j]; tgen[i] = gen[j];
         }
         /* 4: uptake, one update per occupied voxel */
-        for (uint32_t i = 0; i < n;) {
-            uint32_t e = i + 1;
-            while (e < n && keys[e] == keys[i]) e++;
-            int32_t vx = vox1(t[0][i], dx, d->dims[0]), vy = vox1(t[1][i], dx, d->dims[1]),
-                    vz = vox1(t[2][i], dx, d->dims[2]);
-            size_t v = (size_t)vx + X * (size_t)vy + XY * (size_t)vz;
-            for (int f = 0; f < d->n_fields; f++) {
-                const onc_field_desc* fd = &d->fields[f];
-                if (fd->uptake == 0.0f || is_fixed(fd, d->dims, vx, vy, vz)) continue;
-                REAL u = fields[f][v] - (REAL)(e - i) * (REAL)fd->uptake;
-                fields[f][v] = u < (REAL)fd->lo ? (REAL)fd->lo : u;
-            }
-            i = e;
-        }
+        if (ORACLE_PLANT != 2) uptake(d, keys, n, t, fields);
         /* 5: forces and integration into px/py/pz (old order no longer needed) */
 #pragma omp parallel for schedule(dynamic, 4096)
         for (uint32_t i = 0; i < n; i++) {
@@ -430,6 +460,7 @@ int FN(oracle_tumor_run)(const onc_sim_desc* d, uint32_t* n_io, uint32_t* next_i
         /* 6: fields */
         for (int f = 0; f < d->n_fields; f++) FN(oracle_stencil7)(fields[f], swept[f], d->dims, &d->fields[f]);
         for (int f = 0; f < d->n_fields; f++) memcpy(fields[f], swept[f], nvox * sizeof(REAL));
+        if (ORACLE_PLANT == 2) uptake(d, keys, n, t, fields);
         /* 7: rule */
 #pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < n; i++) {

@@ -14,6 +14,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboncooracle.so")
+# A planted-mistake build (tests/test_oracle_mutants.py) is loaded in place of
+# the oracle through this variable; it is never set otherwise.
+_PLANTED = os.environ.get("ONCO_ORACLE_PLANTED_LIB")
 
 _lib = None
 
@@ -26,9 +29,12 @@ def build() -> str:
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        L = C.CDLL(LIB_PATH)
+        if _PLANTED:
+            L = C.CDLL(_PLANTED)
+        else:
+            if not os.path.exists(LIB_PATH):
+                build()
+            L = C.CDLL(LIB_PATH)
         P = C.c_void_p
         u32 = C.c_uint32
         L.oracle_morton30.restype = u32
