@@ -186,7 +186,8 @@ int onc_sim_set_step(onc_sim* sim, uint32_t step);
 /* metric rules: runs the module's onc_metric_main / onc_metric_final kernels
    (translated @ol.metric_rule methods) over the sim's committed cells and
    writes the nvals reduced metric words (binary32 / int32 / uint32 bits, in
-   the metrics class's field order, nvals <= 64) to out (host memory). Sums
+   the metrics class's field order, nvals <= 64; ONC_ERR_INVALID unless it
+   equals the module's onc_metric_nvals word) to out (host memory). Sums
    in a fixed order set by the cell count (ONC_METRIC_CPT in onc_device.h),
    so binary32 metrics are reproducible; integer metrics wrap mod 2^32.
    (replaces the atomicAdd metric kernels of _shader_builder.py:193-204 and

@@ -14,13 +14,11 @@ Differences, each a deterministic-spec decision:
 """
 from __future__ import annotations
 
-import inspect
-import typing
 from typing import Any, Optional, Type
 
 from ._containers import AgentList, Grid
 from ._errors import CompilationError
-from ._types import Cell, Chemistry, Simulation, Tissue
+from ._types import Chemistry, Simulation, Tissue
 
 
 class Engine:
@@ -143,7 +141,9 @@ class Engine:
         for lst in (self._tissue_rules, self._chemistry_rules, self._cell_rules, self._metric_rules):
             lst.sort(key=lambda m: order.get(m.__name__, 1 << 30))
         from .hip._backend import HipBackend
-        self._backend_impl = HipBackend(self)
+        # the reference passes its validator's context here (core/runtime/_engine.py:153-157);
+        # this package has no separate validator: the translator checks the rules
+        self._backend_impl = HipBackend(self, None)
         self._backend_impl.compile(sim_instance)
 
     # ---- execution ----------------------------------------------------------------------
@@ -168,23 +168,6 @@ class Engine:
 
     def sync_to_device(self) -> None:
         self._backend_impl.sync_to_device()
-
-    def _resolve_container_for(self, bound_method: Any) -> Any:
-        try:
-            params = list(inspect.signature(bound_method).parameters.values())
-            if not params:
-                return ()
-            ann = typing.get_type_hints(bound_method).get(params[0].name, params[0].annotation)
-            if isinstance(ann, type):
-                if issubclass(ann, Cell):
-                    return self.cells
-                if issubclass(ann, Chemistry):
-                    return self.chemistry if self.chemistry is not None else ()
-                if issubclass(ann, Tissue):
-                    return self.tissue if self.tissue is not None else ()
-        except (ValueError, TypeError, NameError):
-            pass
-        return ()
 
     def __repr__(self) -> str:
         return f"Engine(backend={self.backend!r}, step={self._step}, cells={len(self.cells)})"

@@ -592,6 +592,17 @@ extern "C" int onc_sim_metrics(onc_sim* s, onc_module* m, int nvals, uint32_t* o
     if (hipModuleGetFunction(&fm, m->mod, "onc_metric_main") != hipSuccess ||
         hipModuleGetFunction(&ff, m->mod, "onc_metric_final") != hipSuccess)
         return fail(ONC_ERR_INVALID, "module has no metric kernels (onc_metric_main, onc_metric_final)");
+    // The kernels write as many words per block as the module's metric class
+    // has fields; the scratch below is sized by nvals, so the two must agree.
+    hipDeviceptr_t gp = nullptr;
+    size_t gbytes = 0;
+    uint32_t nv = 0;
+    if (hipModuleGetGlobal(&gp, &gbytes, m->mod, "onc_metric_nvals") != hipSuccess || gbytes != sizeof(nv))
+        return fail(ONC_ERR_INVALID, "module has no onc_metric_nvals word");
+    CK(hipMemcpy(&nv, gp, sizeof(nv), hipMemcpyDeviceToHost));
+    if ((uint32_t)nvals != nv)
+        return fail(ONC_ERR_INVALID, "nvals (" + std::to_string(nvals) + ") differs from the module's metric count (" +
+                                         std::to_string(nv) + ")");
     onc_dstate st;
     CKR(sim_read_state(s, &st));
     onc_ctx* c = s->ctx;
@@ -657,6 +668,16 @@ extern "C" int onc_sim_set_step(onc_sim* s, uint32_t step) {
     CK(hipSetDevice(s->ctx->device));
     onc_dstate st;
     CKR(sim_read_state(s, &st));
+    if (step < st.step) {
+        // A bin entry is valid while its epoch equals the running step + 1, and
+        // only occupied keys are rewritten each step. Entries left by the steps
+        // being undone carry epochs that the coming steps reuse, so they would
+        // read as valid bins of voxels the restored cells do not occupy: clear
+        // the table, as at creation.
+        hipError_t e = onc_launch_fill_u32(s->dsim.bins, 0xFFFFFFFFu, 4ull * s->table, s->ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+        if (e != hipSuccess) return fail(ONC_ERR_HIP, std::string("set_step: ") + hipGetErrorString(e));
+    }
     st.step = step;
     CK(hipMemcpy(s->dstate, &st, sizeof(st), hipMemcpyHostToDevice));
     return ONC_OK;

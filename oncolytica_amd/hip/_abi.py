@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import threading
+import weakref
 
 import numpy as np
 
@@ -220,30 +221,35 @@ class Context:
         return v.value
 
 
+def _free_pinned(ctx: "Context", p: int) -> None:
+    # After the context is destroyed the pages stay mapped until the process ends.
+    if ctx.h:
+        lib().onc_host_free(ctx.h, _P(p))
+
+
 class PinnedBuffer:
-    """Page-locked host memory (onc_host_alloc) viewed as a numpy array."""
+    """Page-locked host memory (onc_host_alloc) viewed as a numpy array.
+
+    The memory lives as long as any numpy view of it: `array` and every view
+    taken from it keep the underlying ctypes buffer alive, and the memory is
+    released by a finalizer of that buffer. free() drops this handle's own
+    references only, so a view kept elsewhere never points at freed memory."""
 
     def __init__(self, ctx: Context, count: int, dtype=np.uint32):
         self.ctx = ctx
         self.dtype = np.dtype(dtype)
         self.count = int(count)
+        nbytes = max(4, self.count * self.dtype.itemsize)
         p = _P()
-        call("onc_host_alloc", ctx.h, max(4, self.count * self.dtype.itemsize), C.byref(p))
+        call("onc_host_alloc", ctx.h, nbytes, C.byref(p))
+        buf = (C.c_char * nbytes).from_address(p.value)
+        weakref.finalize(buf, _free_pinned, ctx, p.value)
         self.p = p
-        buf = (C.c_char * max(4, self.count * self.dtype.itemsize)).from_address(p.value)
         self.array = np.frombuffer(buf, dtype=self.dtype, count=self.count)
 
     def free(self) -> None:
-        if self.p:
-            self.array = None
-            lib().onc_host_free(self.ctx.h, self.p)
-            self.p = None
-
-    def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        self.array = None
+        self.p = None
 
 
 class DeviceBuffer:

@@ -60,6 +60,10 @@ def _close_contexts() -> None:
 
 _NP = {COL_F32: np.float32, COL_I32: np.int32, COL_U32: np.uint32}
 
+# no built-in forces unless the engine sets them (Engine.setup_mechanics)
+NO_MECHANICS = {"k_rep": 0.0, "k_adh": 0.0, "mobility": 0.25, "max_disp": 0.25, "interaction_radius": None,
+                "cell_radius": None}
+
 
 def _get_path(obj, path: str):
     for p in path.split("."):
@@ -67,10 +71,60 @@ def _get_path(obj, path: str):
     return obj
 
 
+def _grid_field(grid, name: str) -> np.ndarray:
+    """One field of a tissue grid, flattened x fastest (the device layout).
+    The package's Grid stores fields as arrays (Grid.flat); the reference's
+    Grid holds one record object per voxel in an [x, y, z] object array
+    (core/utils/_containers.py:17-51)."""
+    flat = getattr(grid, "flat", None)
+    if callable(flat):
+        return flat(name)
+    objs = grid._data
+    vals = np.empty(objs.shape, dtype=np.float64)
+    for idx, o in np.ndenumerate(objs):
+        vals[idx] = getattr(o, name)
+    return np.ascontiguousarray(vals.transpose(2, 1, 0)).reshape(-1)
+
+
+def _set_grid_field(grid, name: str, values: np.ndarray) -> None:
+    flat = getattr(grid, "flat", None)
+    if callable(flat):
+        dst = flat(name)
+        dst[:] = values.astype(dst.dtype)
+        return
+    objs = grid._data
+    X, Y, Z = objs.shape
+    v = values.reshape(Z, Y, X).transpose(2, 1, 0)
+    for idx, o in np.ndenumerate(objs):
+        setattr(o, name, type(getattr(o, name))(v[idx]))
+
+
+def _columnar(cells) -> Optional[dict]:
+    """The columns of a columnar AgentList (this package's set_columns), or
+    None for a list of cell objects (the reference's AgentList,
+    core/utils/_containers.py:85-123)."""
+    fn = getattr(cells, "columns", None)
+    cols = fn() if callable(fn) else None
+    return cols if cols is not None and not cells._data else None
+
+
 class HipBackend:
-    def __init__(self, engine: Any) -> None:
+    """Constructed as the reference constructs GpuBackend, (engine, ctx)
+    (core/runtime/_engine.py:153-161): `ctx` is the validator's context,
+    which the reference's GPU compiler reads; the translator here does its own
+    checks and keeps it only for callers that look for it. Of the engine it
+    reads the reference Engine's attributes (core/runtime/_engine.py:26-55):
+    max_agents, tissue_shape, tissue_voxel_size, cell_diameter, cells,
+    tissue, _sim, _metrics and the rule lists. device, seed, key_format and
+    mechanics are this package's Engine's additions and default to device 0,
+    seed 0, the 30-bit key format where the grid allows it, and no built-in
+    forces (the reference has none)."""
+
+    def __init__(self, engine: Any, ctx: Any = None) -> None:
         self.engine = engine
-        self.ctx = context(engine.device)
+        self.validation_ctx = ctx
+        self.device = int(getattr(engine, "device", 0))
+        self._hctx: Optional[_abi.Context] = None
         self.sim: Optional[Sim] = None
         self.module: Optional[Module] = None
         self.schema: Optional[CellSchema] = None
@@ -78,6 +132,13 @@ class HipBackend:
         self.source: str = ""
         self._next_id = 0
         self.metric_fields: list = []  # (name, "f32" / "i32" / "u32") per metric word
+
+    @property
+    def ctx(self) -> _abi.Context:
+        """The HIP context of the engine's device (created on first use)."""
+        if self._hctx is None:
+            self._hctx = context(self.device)
+        return self._hctx
 
     # ---- compile ---------------------------------------------------------------------
     def compile(self, sim_instance: Any) -> None:
@@ -101,12 +162,14 @@ class HipBackend:
     def _desc(self, sim_instance) -> SimDesc:
         e = self.engine
         d = SimDesc()
-        d.dims[:] = list(e.tissue_shape)
+        shape = tuple(int(v) for v in e.tissue_shape)
+        d.dims[:] = list(shape)
         d.dx = e.tissue_voxel_size
-        d.key_format = e.key_format
+        d.key_format = int(getattr(e, "key_format", 30 if max(shape) <= 1024 else 32))
         d.capacity = e.max_agents
-        d.seed = e.seed
-        m = e.mechanics
+        d.seed = int(getattr(e, "seed", 0))
+        m = dict(NO_MECHANICS)
+        m.update(getattr(e, "mechanics", None) or {})
         d.interaction_radius = m["interaction_radius"] if m["interaction_radius"] is not None else \
             min(e.tissue_voxel_size, e.cell_diameter)
         d.cell_radius = m["cell_radius"] if m["cell_radius"] is not None else 0.5 * e.cell_diameter
@@ -240,10 +303,11 @@ class HipBackend:
     def sync_to_device(self) -> None:
         e = self.engine
         cells = e.cells
-        if cells.columns() is not None and not cells._data:
-            cols = self._columns_from_arrays(cells.columns())
+        columnar = _columnar(cells)
+        if columnar is not None:
+            cols = self._columns_from_arrays(columnar)
         else:
-            cells._remove_dead()
+            cells._data = [a for a in cells._data if getattr(a, "_alive", True)]
             cols = self._columns_from_objects(cells._data)
         n = len(cols[0])
         if n > e.max_agents:
@@ -253,7 +317,7 @@ class HipBackend:
         self._next_id = next_id
         if e.tissue is not None:
             for f, (name, t, _) in enumerate(self.tissue.fields):
-                self.sim.upload_field(f, e.tissue.flat(name).astype(np.float32))
+                self.sim.upload_field(f, _grid_field(e.tissue, name).astype(np.float32))
 
     def download_columns(self) -> dict:
         """Cells as numpy columns keyed by field name ('pos' is (n, 3))."""
@@ -276,7 +340,7 @@ class HipBackend:
         cols = self.download_columns()
         n_, nid, _ = self.sim.counts()
         self._next_id = nid
-        if e.cells.columns() is not None and not e.cells._data:
+        if _columnar(e.cells) is not None:
             e.cells.set_columns(cols)
         else:
             cls = self.schema.cls
@@ -306,8 +370,7 @@ class HipBackend:
             e.cells._data = objs
         if e.tissue is not None:
             for f, (name, t, _) in enumerate(self.tissue.fields):
-                arr = self.sim.download_field(f)
-                e.tissue.flat(name)[:] = arr.astype(e.tissue.flat(name).dtype)
+                _set_grid_field(e.tissue, name, self.sim.download_field(f))
 
     def close(self) -> None:
         if self.sim is not None:

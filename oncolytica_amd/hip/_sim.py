@@ -63,8 +63,14 @@ class Sim:
         self.module = module
 
     def set_params(self, params: Sequence[float]) -> None:
+        """Rule parameters for the next steps. The host descriptor follows the
+        device copy (as onc_sim_set_params updates it), so a bundle exported
+        afterwards holds the parameters the sim runs with."""
         arr = (C.c_float * len(params))(*params)
         call("onc_sim_set_params", self.h, arr, len(params))
+        for k, v in enumerate(arr):
+            self.desc.params[k] = v
+        self.desc.n_params = max(int(self.desc.n_params), len(params))
 
     def upload_cells(self, cols: Sequence[np.ndarray], next_id: Optional[int] = None) -> None:
         if len(cols) != self.n_cols:

@@ -1338,7 +1338,9 @@ class Translator:
         if (t == 0u) a.out[{k}u] = s_v[0];
         __syncthreads();
     }}""")
-        return (f"""extern "C" __global__ __launch_bounds__(256) void onc_metric_main(onc_metric_args a) {{
+        return (f"""extern "C" __device__ onc_u32 onc_metric_nvals = {nv}u;  // words per block and in out (onc_sim_metrics checks)
+
+extern "C" __global__ __launch_bounds__(256) void onc_metric_main(onc_metric_args a) {{
     __shared__ onc_u32 s_v[256];
     const onc_dsim& S = *a.S;
     const onc_u32 t = threadIdx.x;

@@ -24,7 +24,8 @@ waiting job has waited longer than `preempt_after` seconds and does not fit,
 the resident jobs with the most steps left are evicted between ticks, each
 to a bundle (oncolytica_amd.bundle: committed cells, fields, step counter),
 until it fits; they wait again in arrival order and are restored from their
-bundles when admitted, continuing bit for bit.
+bundles when admitted, continuing bit for bit. The memory freed is held for
+the job it was freed for: it is admitted before any evicted job.
 """
 from __future__ import annotations
 
@@ -164,6 +165,7 @@ class Server:
         staged: list[Job] = []     # admitted: resident on the device, joining the batch at the next tick
         active: list[Job] = []     # in the batch
         used = 0
+        reserved: Optional[Job] = None  # the job that preemption freed memory for
         ticks = graph_ticks = peak_res = peak_bytes = evictions = 0
         total_cu = 0
         times = {"admit_s": 0.0, "retire_s": 0.0, "launch_s": 0.0, "gpu_wait_s": 0.0}
@@ -175,14 +177,25 @@ class Server:
                 waiting.append(pending.pop(0))
 
         def admit():
-            # arrival order, first fit within the budget and the slots
-            nonlocal used
+            # arrival order, first fit within the budget and the slots; memory
+            # freed by preemption is held for the job it was freed for, which
+            # is admitted before any other (else an evicted job that arrived
+            # earlier would take the space back, be evicted again at the next
+            # tick, and so on, while the job it was evicted for still waits)
+            nonlocal used, reserved
             ta = clock()
-            for j in list(waiting):
+            order = list(waiting)
+            if reserved is not None:
+                order = [reserved] + [w for w in order if w is not reserved]
+            for j in order:
                 if len(active) + len(staged) >= self.max_jobs:
                     break
                 if used + j.footprint > self.budget:
+                    if j is reserved:
+                        break
                     continue
+                if j is reserved:
+                    reserved = None
                 if j.bundle is not None:
                     # an evicted job: restored with its cells, fields and step counter
                     j.sim, _, _ = B.load_bundle(self.ctx, j.bundle)
@@ -261,6 +274,7 @@ class Server:
                 if fresh and self.evict_dir is not None:
                     head = min(fresh, key=lambda w: w.arrival)
                     if (clock() - t0) - head.arrival > self.preempt_after and used + head.footprint > self.budget:
+                        reserved = head
                         for v in sorted(active, key=lambda a: (-a.left, a.id)):
                             if used + head.footprint <= self.budget:
                                 break

@@ -105,3 +105,59 @@ def test_empty_sim_metrics_and_bundle(ctx, tmp_path):
     finally:
         mod.close()
         sim.close()
+
+
+def test_bundle_after_update_params_continues_bit_for_bit(ctx, tmp_path):
+    """Parameters changed with Engine.update_params reach the bundle: the
+    restored sim divides with the new division age, like the original."""
+    w = W.plain_c1()
+    cols = W.initial_cells(w)
+    eng = ol.Engine(backend="hip", max_agents=1 << 16, seed=w.seed)
+    eng.setup_geometry((64, 64, 64), 1.0, cell_diameter=1.0)
+    eng.setup_mechanics(k_rep=1.0, k_adh=0.0, mobility=0.25, max_disp=0.25, interaction_radius=1.0,
+                        cell_radius=0.5)
+    eng.setup_tissue(SpheroidTissue)
+    eng.cells.set_columns({"pos": np.stack(cols[:3], 1), "_id": cols[3], "age": cols[4], "generation": cols[5]})
+    eng.load_model(Spheroid())
+    eng.run_steps(10)
+    eng.update_params(div_age=12.0)
+    eng.run_steps(10)
+    be = eng._backend_impl
+    path = be.export_bundle(str(tmp_path / "p"))
+    r, mod, cfg = B.load_bundle(ctx, path)
+    try:
+        assert 12.0 in cfg["desc"]["params"] and 24.0 not in cfg["desc"]["params"]
+        _same(be.sim, r, "restored")
+        be.sim.step(20)
+        r.step(20)
+        _same(be.sim, r, "20 + 20 steps")
+    finally:
+        r.close()
+        mod.close()
+
+
+def test_metrics_reject_a_word_count_other_than_the_modules(ctx):
+    """onc_sim_metrics takes exactly the module's metric count: fewer words
+    would let its kernels write past the scratch it sizes, more would return
+    words nobody wrote."""
+    from oncolytica_amd.hip import _abi
+    from oncolytica_amd.hip._sim import Module
+    from oncolytica_amd.hip._translator import CellSchema, TissueSchema, Translator
+    from oncolytica_amd.spheroid import SpheroidMetrics, TumorCell
+    w = W.plain_c1()
+    sim = Sim(ctx, w.desc())
+    model = Spheroid()
+    tr = Translator(model, CellSchema.of(TumorCell), TissueSchema.of(SpheroidTissue), model._params)
+    mod = Module(ctx, tr.translate([model.lifecycle], [model.count], SpheroidMetrics))
+    try:
+        sim.upload_cells(W.initial_cells(w))
+        nv = len(tr.metric_fields)
+        assert nv == 4
+        for bad in (nv - 1, nv + 1):
+            with pytest.raises(_abi.OncError, match="metric count"):
+                sim.metrics(mod, bad)
+        got = sim.metrics(mod, nv)
+        assert int(got[0]) == 4096
+    finally:
+        mod.close()
+        sim.close()

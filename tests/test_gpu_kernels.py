@@ -271,5 +271,11 @@ def test_pinned_host_buffers_round_trip(ctx):
     _abi.call("onc_h2d", ctx.h, dev.p, src.p, 4 * n)
     _abi.call("onc_d2h", ctx.h, dst.p, dev.p, 4 * n)
     np.testing.assert_array_equal(dst.array, src.array)
-    src.free()
+    # a view taken from the buffer keeps its memory after the handle is freed
+    view = dst.array[10:20]
     dst.free()
+    del dst
+    import gc
+    gc.collect()
+    np.testing.assert_array_equal(view, np.arange(10, 20, dtype=np.uint32) * np.uint32(2654435761))
+    src.free()

@@ -159,3 +159,37 @@ def test_voxel_over_force_walk_limit_is_reported(ctx):
             sim.sync()
     finally:
         sim.close()
+
+
+def test_set_step_backwards_then_new_state_matches_oracle(ctx):
+    """Run 30 steps, set the step counter back to 10 and upload the step-10
+    state without its cells at x < 32: the next 20 steps equal the oracle's
+    from that state. Bin entries written by the first run's steps 10..29
+    carry the epochs the next 20 steps use, and those of the removed half lie
+    in the neighbourhoods of the cells left at x ~ 32: they must read as empty."""
+    w = W.plain_c1()
+    d = w.desc()
+    cols = W.initial_cells(w)
+    ref = O.TumorState(d, cols)
+    ref.run(10)
+    keep = ref.columns()[0] >= np.float32(32.0)
+    half = O.TumorState(d, [c[keep] for c in ref.columns()], fields=ref.fields, next_id=ref.next_id.value)
+    half.step.value = 10
+    assert 0 < keep.sum() < len(keep)
+    sim = Sim(ctx, d)
+    try:
+        sim.upload_cells(cols)
+        sim.step(30)
+        sim.sync()
+        print("set_step: 30 steps run, rewinding to 10", flush=True)
+        sim.set_step(10)
+        sim.upload_cells(half.columns(), next_id=half.next_id.value)
+        for f in range(d.n_fields):
+            sim.upload_field(f, half.fields[f])
+        assert sim.counts()[2] == 10
+        sim.step(20)
+        sim.sync()
+        half.run(20)
+        compare(30, sim.download_cells(), [sim.download_field(f) for f in range(d.n_fields)], half)
+    finally:
+        sim.close()

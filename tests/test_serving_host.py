@@ -154,3 +154,41 @@ def test_autotune_signature_groups_capacities():
     c = W.large_c2().desc()
     assert _autotune.signature("gfx950", [a, c]) == _autotune.signature("gfx950", [c, a])
     assert _autotune.signature("gfx950", [a]) != _autotune.signature("gfx950", [c])
+
+
+class FakeBundles:
+    """Stand-in for oncolytica_amd.bundle: remembers an evicted sim's steps."""
+
+    def __init__(self):
+        self.saved = {}
+        self.log = []
+
+    def export_bundle(self, sim, path):
+        self.saved[path] = sim.steps
+        self.log.append(("export", path))
+
+    def load_bundle(self, ctx, path):
+        s = FakeSim(ctx, None)
+        s.steps = s.base = self.saved.pop(path)
+        self.log.append(("load", path))
+        return s, None, {}
+
+
+def test_preemption_admits_the_job_it_evicted_for(fake, monkeypatch, tmp_path):
+    """Job 0 holds the budget; job 1 arrives later and cannot fit. After the
+    wait limit job 0 is evicted, and job 1 (not job 0, which arrived first
+    and fits again) takes the freed memory: one eviction, no ping-pong."""
+    clock, batches = fake
+    bundles = FakeBundles()
+    monkeypatch.setattr(serving, "B", bundles)
+    monkeypatch.setattr(FakeSim, "base", 0, raising=False)
+    monkeypatch.setattr(serving, "_cu", lambda sim: sim.steps - sim.base)
+    jobs = _jobs([(0.0, 6, 300), (0.001, 6, 100)])
+    rep = serving.Server(None, hbm_budget=1000, max_tick_steps=16, evict_dir=str(tmp_path),
+                         preempt_after=0.01).run(jobs, clock=clock, sleep=clock.sleep)
+    assert rep.evictions == 1 and jobs[0].evictions == 1 and jobs[1].evictions == 0
+    assert [k for k, _ in bundles.log] == ["export", "load"]
+    # job 1 ran to its end before job 0 came back
+    assert jobs[1].finished <= jobs[0].finished
+    assert [j.cell_updates for j in jobs] == [300, 100]
+    assert FakeSim.live == 0
