@@ -134,7 +134,9 @@ def test_stress_16m_512_steps_bit_exact_and_drift(ctx):
 def test_served_jobs_full_size_match_the_oracle(ctx):
     jobs = serving.poisson_jobs(4.0, 3.0, seed=3)[:8]
     for i, j in enumerate(jobs):
-        j.horizon = 6 + 2 * i
+        # arrivals 20 ms apart and horizons of 20..48 steps: later jobs join a running batch
+        j.arrival = 0.02 * i
+        j.horizon = 20 + 4 * i
     assert max(j.workload.n_cells for j in jobs) > 2_000_000
     rep = serving.Server(ctx, hbm_budget=200 << 30, max_tick_steps=4, keep_results=True,
                          log=lambda m: print(m, flush=True)).run(jobs)
