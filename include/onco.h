@@ -261,6 +261,13 @@ int onc_forces(onc_ctx* ctx, const onc_sim_desc* desc, const float* px, const fl
 /* exclusive prefix sum; *total (device pointer, may be NULL) receives the sum
  * (replaces template.wgsl:192-329, Blelloch 3-level scan) */
 int onc_exscan_u32(onc_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total);
+/* onc_exscan_u32 without any allocation: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Its look-back status
+ * and error words live in a fixed 2 MiB + 256 B device array per GPU, so n is
+ * limited to 2^30 (ONC_ERR_INVALID beyond); calls that overlap on one GPU
+ * share that array and race. Same results as onc_exscan_u32. */
+int onc_exscan_u32_noalloc(onc_ctx* ctx, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total);
 /* bin start/end for every occupied key of sorted keys; other entries untouched */
 int onc_bins(onc_ctx* ctx, const uint32_t* sorted_keys, uint32_t n, uint32_t* start,
              uint32_t* end);

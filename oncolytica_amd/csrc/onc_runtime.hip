@@ -56,8 +56,10 @@ struct onc_ctx {
     std::string arch;
     hipDeviceProp_t prop{};
     // scratch for pure kernels
-    std::vector<void*> scratch;
+    std::vector<void*> scratch_bufs;
     size_t scratch_bytes = 0;
+    // this device's copy of the file-scope `scratch` array (onc_exscan_u32_noalloc)
+    void* noalloc_scratch = nullptr;
     // optional GPU timing of the pure kernels (onc_ctx_timing)
     bool timing = false;
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -144,6 +146,18 @@ struct onc_batch {
     uint64_t phase_n[ONC_PHASES] = {0};
 };
 
+// Scratch of onc_exscan_u32_noalloc, for callers that cannot allocate: 256
+// bytes holding the error word, then one 8-byte look-back status word per
+// 4096-element tile of the largest input it accepts (2^30 elements), 2 MiB
+// + 256 B in all. Every device that runs the library holds its own copy
+// until the process ends. Host code never uses this name as an address: each
+// context takes its device's address with hipGetSymbolAddress at creation
+// (onc_ctx::noalloc_scratch) and passes that to the kernels.
+#define ONC_NOALLOC_SCAN_MAX_N (1u << 30)
+#define ONC_NOALLOC_ERR_WORDS 64u
+#define ONC_NOALLOC_SCRATCH_BYTES (4u * ONC_NOALLOC_ERR_WORDS + 8u * (ONC_NOALLOC_SCAN_MAX_N / ONC_SCAN_MIN_TILE))
+static __device__ __align__(256) unsigned char scratch[ONC_NOALLOC_SCRATCH_BYTES];
+
 // ---------------------------------------------------------------------------
 // errors / version
 // ---------------------------------------------------------------------------
@@ -175,6 +189,13 @@ extern "C" int onc_ctx_create(int device, onc_ctx** out) {
         delete c;
         return fail(ONC_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
+    // this device's address of the no-alloc scratch (device current: set above)
+    e = hipGetSymbolAddress(&c->noalloc_scratch, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !c->noalloc_scratch) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return fail(ONC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
     *out = c;
     return ONC_OK;
 }
@@ -183,7 +204,7 @@ extern "C" int onc_ctx_destroy(onc_ctx* c) {
     if (!c) return ONC_OK;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    for (void* p : c->scratch) hipFree(p);
+    for (void* p : c->scratch_bufs) hipFree(p);
     if (c->t0) hipEventDestroy(c->t0);
     if (c->t1) hipEventDestroy(c->t1);
     hipStreamDestroy(c->stream);
@@ -1246,14 +1267,14 @@ extern "C" int onc_batch_phase_times(onc_batch* b, double* us, uint64_t* launche
 static int ctx_scratch(onc_ctx* c, size_t bytes, void** out) {
     if (bytes > c->scratch_bytes) {
         CK(hipStreamSynchronize(c->stream));
-        for (void* p : c->scratch) hipFree(p);
-        c->scratch.clear();
+        for (void* p : c->scratch_bufs) hipFree(p);
+        c->scratch_bufs.clear();
         void* p = nullptr;
         if (hipMalloc(&p, bytes) != hipSuccess) return fail(ONC_ERR_NOMEM, "scratch allocation");
-        c->scratch.push_back(p);
+        c->scratch_bufs.push_back(p);
         c->scratch_bytes = bytes;
     }
-    *out = c->scratch[0];
+    *out = c->scratch_bufs[0];
     return ONC_OK;
 }
 
@@ -1477,6 +1498,35 @@ extern "C" int onc_exscan_u32(onc_ctx* c, const uint32_t* in, uint32_t* out, uin
     CK(hipStreamSynchronize(c->stream));
     if (herr) return fail(ONC_ERR_DEVICE, "scan look-back wait expired");
     return tread(c);
+}
+
+// The same scan with its look-back status and error words in the device's
+// `scratch` array instead of the context's allocated scratch: no hipMalloc,
+// hipFree or allocating library call. Every call on a device shares that
+// device's array, so calls that overlap on one device (several threads or
+// streams) race; the words are cleared by a fill kernel at each call.
+extern "C" int onc_exscan_u32_noalloc(onc_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n,
+                                      uint32_t* total) {
+    if (!c || (n && (!in || !out))) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (n > ONC_NOALLOC_SCAN_MAX_N)
+        return fail(ONC_ERR_INVALID, "n exceeds the no-alloc scan's limit of 2^30 elements");
+    if (!c->noalloc_scratch) return fail(ONC_ERR_STATE, "context has no scratch address");
+    CK(hipSetDevice(c->device));
+    if (n == 0) {
+        if (total) CK(onc_launch_fill_u32(total, 0u, 1, c->stream));
+        CK(hipStreamSynchronize(c->stream));
+        return ONC_OK;
+    }
+    const size_t tiles = (n + ONC_SCAN_MIN_TILE - 1) / ONC_SCAN_MIN_TILE;
+    onc_u32* err = (onc_u32*)c->noalloc_scratch;
+    onc_u32* status = err + ONC_NOALLOC_ERR_WORDS;  // 8-byte words, 256-byte aligned
+    CK(onc_launch_fill_u32(err, 0u, ONC_NOALLOC_ERR_WORDS + 2 * tiles, c->stream));
+    CK(onc_launch_exscan(in, out, n, total, status, err, c->stream));
+    onc_u32 herr = 0;
+    CK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    if (herr) return fail(ONC_ERR_DEVICE, "scan look-back wait expired");
+    return ONC_OK;
 }
 
 extern "C" int onc_bins(onc_ctx* c, const uint32_t* keys, uint32_t n, uint32_t* start, uint32_t* end) {

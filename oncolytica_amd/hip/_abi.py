@@ -131,6 +131,7 @@ _PROTOS = {
     "onc_sort_pairs": (C.c_int, [_P, _P, _P, _u32, C.c_int, _P, _P]),
     "onc_forces": (C.c_int, [_P, C.POINTER(SimDesc), _P, _P, _P, _u32, _P, _P, _P]),
     "onc_exscan_u32": (C.c_int, [_P, _P, _P, _u32, _P]),
+    "onc_exscan_u32_noalloc": (C.c_int, [_P, _P, _P, _u32, _P]),
     "onc_bins": (C.c_int, [_P, _P, _u32, _P, _P]),
     "onc_stencil7": (C.c_int, [_P, _P, _P, _i32p, C.POINTER(FieldDesc)]),
     "onc_philox": (C.c_int, [_P, _P, _u32, C.c_uint64, _P]),
