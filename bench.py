@@ -87,7 +87,7 @@ def cpu_baseline(w, steps: int, steps_1t: int):
                       f"({w.n_cells:,} cells, {w.dims[0]}^3, {len(w.fields)} fields), "
                       f"oracle/onco_oracle_real.c binary32 on {threads} OpenMP threads, {dt:.1f} s",
             "value_1thread": v1, "sample_1thread": f"{steps_1t} steps on 1 thread, {dt1:.1f} s",
-            "host_cpu": _cpu_model()}
+            "host_cpu": _cpu_model(), "host": _host_cpus()}
 
 
 def _cpu_model() -> str:
@@ -100,12 +100,50 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def _host_cpus() -> dict:
+    """Logical CPUs of the host (nproc: those this process may run on; the
+    machine's total) and lscpu's socket / core / thread counts."""
+    import subprocess
+    out = {"nproc": len(os.sched_getaffinity(0)), "logical_cpus": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                     "On-line CPU(s) list", "Vendor ID", "CPU max MHz"):
+                out["lscpu " + k] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return out
+
+
 def _sim_cu(sim) -> int:
     import ctypes as C
     from oncolytica_amd.hip import _abi
     v = C.c_uint64()
     _abi.call("onc_sim_cell_updates", sim.h, C.byref(v))
     return int(v.value)
+
+
+class Marker:
+    """A one-element Philox launch (k_philox, which nothing else in the step
+    workload runs) before and after the timed region, so that a kernel trace
+    of this command can pick out the timed region's dispatches."""
+
+    def __init__(self, ctx):
+        import numpy as np
+        self.ctx = ctx
+        self.a = ctx.upload(np.zeros(4, np.uint32))
+        self.b = ctx.alloc(16)
+
+    def __call__(self):
+        from oncolytica_amd.hip import _abi
+        _abi.call("onc_philox", self.ctx.h, self.a.p, 1, 1, self.b.p)
+
+    def free(self):
+        self.a.free()
+        self.b.free()
 
 
 def kernel_rooflines(ctx):
@@ -592,6 +630,68 @@ def summarize(d: str, workload: str = "large_1m_256", outdir: str = None):
     print(f"wrote {tj} and {md}")
 
 
+# the driver's command under a kernel trace (`--summarize-timed DIR` reduces
+# it). The serving leg runs after the timed region and is left out: under the
+# kernel trace the process took a host-side SIGSEGV inside the HIP runtime
+# during that leg, which runs to completion without the profiler.
+DRIVER_ARGS = "--gpus 1 --steps 20 --warmup 5"
+TIMED_RUN = ("rocprofv3 --kernel-trace --stats -d {d}/kt -o run -- python3 bench.py " + DRIVER_ARGS +
+             " --serve-seconds 0 > {d}/bench.json")
+
+
+def summarize_timed(d: str, outdir: str = None, workload: str = "large_1m_256"):
+    """Reduce a TIMED_RUN (its trace under d/kt, its JSON line in
+    d/bench.json) to profiles/timed_region_<workload>.md: every kernel
+    dispatched between the two Marker launches (the timed region), and the
+    dominant kernel's traced average duration beside the event-timed one the
+    JSON line's roofline uses."""
+    rows = _rocpd(os.path.join(d, "kt")).execute("select name, start, end from kernels order by start").fetchall()
+    base = [r[0].split("(")[0].split()[-1].split("<")[0] for r in rows]
+    marks = [i for i, b in enumerate(base) if b == "k_philox"]
+    if len(marks) < 2:
+        raise SystemExit("the trace has no pair of timed-region markers (k_philox)")
+    m0, m1 = marks[0], marks[1]
+    line = None
+    for ln in open(os.path.join(d, "bench.json")):
+        if ln.startswith("{"):
+            line = json.loads(ln)
+    roof = line["roofline"]
+    stats = {}
+    for (name, st, en), b in zip(rows[m0 + 1:m1], base[m0 + 1:m1]):
+        n, tot, mn, mx = stats.get(b, (0, 0, float("inf"), 0))
+        du = en - st
+        stats[b] = (n + 1, tot + du, min(mn, du), max(mx, du))
+    dom = "k_" + roof["kernel"]
+    n, tot, _, _ = stats[dom]
+    avg_us = tot / n / 1e3
+    frac_trace = roof["alg_bytes_per_launch"] / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS
+    region_ms = (rows[m1][1] - rows[m0][2]) / 1e6
+    lines = [f"# Timed region of the benchmark command, {workload}", "",
+             "Kernel trace of the benchmark command itself:", "",
+             "    export TMPDIR=/tmp; " + TIMED_RUN.format(d=d), "",
+             f"The timed region is the span between the two one-element `k_philox` marker launches "
+             f"({line['steps']} steps, {region_ms:.2f} ms from the end of the first marker to the start of the second; "
+             f"the JSON line reports {line['ms_per_step'] * line['steps']:.2f} ms on the host clock).", "",
+             "| kernel | calls | avg us | min us | max us | total ms |", "|---|---|---|---|---|---|"]
+    for b, (n_, t_, mn, mx) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"| `{b}` | {n_} | {t_ / n_ / 1e3:.2f} | {mn / 1e3:.2f} | {mx / 1e3:.2f} | {t_ / 1e6:.3f} |")
+    lines += ["", f"Dominant kernel `{dom}`: {n} launches in the region, traced average {avg_us:.2f} us; "
+                  f"the JSON line's event-timed average over {roof['launches_timed']} sampled launches is "
+                  f"{roof['avg_launch_us']:.2f} us (ratio {roof['avg_launch_us'] / avg_us:.3f}).",
+              f"Algorithmic bytes per launch {roof['alg_bytes_per_launch'] / 1e6:.1f} MB: "
+              f"{roof['alg_bytes_per_launch'] / (avg_us * 1e-6) / 1e9:.0f} GB/s on the traced average, "
+              f"frac {frac_trace:.3f} of 8 TB/s (the JSON line: {roof['frac']:.3f})."]
+    for part, v in (roof.get("parts") or {}).items():
+        lines.append(f"Its {part} part alone (`{v['kernel']}`, separate-launch pass): {v['avg_launch_us']:.2f} us, "
+                     f"{v['achieved']:.0f} GB/s, frac {v['frac']:.3f} ({v['bound']}).")
+    outdir = outdir or os.path.join(ROOT, "profiles")
+    os.makedirs(outdir, exist_ok=True)
+    md = os.path.join(outdir, f"timed_region_{workload}.md")
+    with open(md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(f"wrote {md}")
+
+
 def summarize_kernels(d: str, outdir: str = None):
     """Reduce the PROFILE_RUNS_KERNELS outputs under `d` to
     profiles/pure_kernels_16m.md: per pure kernel, its traced duration and
@@ -642,6 +742,9 @@ def main():
     if len(sys.argv) >= 3 and sys.argv[1] == "--summarize":
         summarize(sys.argv[2], outdir=sys.argv[3] if len(sys.argv) > 3 else None)
         return
+    if len(sys.argv) >= 3 and sys.argv[1] == "--summarize-timed":
+        summarize_timed(sys.argv[2], outdir=sys.argv[3] if len(sys.argv) > 3 else None)
+        return
     if len(sys.argv) >= 3 and sys.argv[1] == "--summarize-kernels":
         summarize_kernels(sys.argv[2], outdir=sys.argv[3] if len(sys.argv) > 3 else None)
         return
@@ -665,15 +768,18 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-kernels", action="store_true",
                     help="skip the pure-kernel rooflines (sort, scan, sweep, Morton at large sizes)")
-    ap.add_argument("--event-every", type=int, default=10,
+    ap.add_argument("--event-every", type=int, default=0,
                     help="in the timed region, HIP events bracket the dominant kernel on every N-th step "
-                         "(an event pair between launches costs GPU time of its own)")
+                         "(an event pair between launches costs GPU time of its own; 0: every 2nd step up "
+                         "to 200 steps, so a short run still samples 10 or more launches, else every 10th)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only the pure-kernel rooflines (one JSON line with their figures)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run a plain 256 MiB copy kernel 5 times first (HBM byte-counter calibration)")
     args = ap.parse_args()
 
+    if args.event_every <= 0:
+        args.event_every = 2 if args.steps <= 200 else 10
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(sims and ensembles would shard by job, one process per GPU)", file=sys.stderr)
@@ -776,10 +882,14 @@ def main():
     batch.profile(True, phases=[dom], every=args.event_every)
     batch.sync()
     cu0 = cell_updates(sim)
+    marker = Marker(ctx)
+    marker()
     t0 = time.perf_counter()
     batch.step(args.steps, graph=False)
     batch.sync()
     t1 = time.perf_counter()
+    marker()
+    marker.free()
     cu1 = cell_updates(sim)
     dom_us, dom_n = batch.phase_times()[dom]
     batch.profile(False)
@@ -841,6 +951,18 @@ def main():
             "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic,
             "alg_bytes_per_launch": dom_bytes, "avg_launch_us": dom_avg,
             "events_every_steps": args.event_every, "launches_timed": dom_n}
+    # the fused launch's two parts, each timed as its own launch in the
+    # separate-launch profiled pass (same steps): the HBM-bound sweep and the
+    # latency-bound force walk
+    parts = {}
+    for part, ph in (("sweep", "stencil"), ("walk", "forces")):
+        row = table_sep.get(ph)
+        if row and row.get("GBps"):
+            parts[part] = {"kernel": "k_stencil" if ph == "stencil" else "k_forces", "avg_launch_us": row["avg_us"],
+                           "alg_bytes_per_launch": row["alg_bytes"], "achieved": row["GBps"],
+                           "frac": row["GBps"] / HBM_PEAK_GBS,
+                           "bound": "hbm" if ph == "stencil" else "gather latency"}
+    roof["parts"] = parts
 
     met = metrics_leg(ctx, desc, cols0, Sim)
     bun = bundle_leg(ctx, desc, cols0, Sim)
