@@ -36,6 +36,7 @@ extern "C" {
 #define ONC_MAX_FIELDS 4
 #define ONC_MAX_COLS 48
 #define ONC_MAX_PARAMS 32
+#define ONC_MAX_CHEM_COLS 16
 
 /* status codes */
 #define ONC_OK 0
@@ -115,6 +116,11 @@ typedef struct {
     int32_t col_types[ONC_MAX_COLS];  /* ONC_COL_* per column */
     int32_t n_params;
     float params[ONC_MAX_PARAMS];     /* rule parameters (Params dataclass values) */
+    /* coarse chemistry grid: max(1, dims / 2) voxels per axis of edge 2 dx
+       (core/utils/_types.py:331-346, _engine.py:75-80); n_chem_cols 32-bit
+       columns of the Chemistry record, all zero at creation */
+    int32_t n_chem_cols;
+    int32_t chem_col_types[ONC_MAX_CHEM_COLS];  /* ONC_COL_* per column */
 } onc_sim_desc;
 
 typedef struct {
@@ -159,7 +165,12 @@ int onc_module_destroy(onc_module* mod);
  *      gpu/runtime/_gpu_backend.py:42-104) ------------------------------- */
 int onc_sim_create(onc_ctx* ctx, const onc_sim_desc* desc, onc_sim** out);
 int onc_sim_destroy(onc_sim* sim);
-/* mod == NULL selects the built-in tumour rule; otherwise kernel `name` of mod */
+/* mod == NULL selects the built-in tumour rule; otherwise kernel `name` of mod.
+   A module that also holds translated @tissue_rule / @chemistry_rule methods
+   (kernels onc_tissue_pass / onc_chem_pass and the words onc_grid_tissue_passes
+   / onc_grid_chem_passes) runs them every step after the diffusion sweep and
+   before the cell rule (replaces the tissue and chemistry dispatches of
+   GpuBackend.run_step, gpu/runtime/_gpu_backend.py:127-142) */
 int onc_sim_set_rule(onc_sim* sim, onc_module* mod, const char* name);
 int onc_sim_set_params(onc_sim* sim, const float* params, int n);
 /* columns: n_cols host pointers, each n 32-bit elements (sync_to_device) */
@@ -168,6 +179,11 @@ int onc_sim_upload_cells(onc_sim* sim, const void* const* cols, uint32_t n, uint
 int onc_sim_download_cells(onc_sim* sim, void* const* cols, uint32_t cap, uint32_t* n);
 int onc_sim_upload_field(onc_sim* sim, int field, const float* host);
 int onc_sim_download_field(onc_sim* sim, int field, float* host);
+/* one column of the chemistry grid (32-bit words, x fastest over its voxels)
+   (the chemistry part of sync_to_device / sync_to_host, _gpu_backend.py:369-371,
+   399-405) */
+int onc_sim_upload_chem(onc_sim* sim, int col, const void* host);
+int onc_sim_download_chem(onc_sim* sim, int col, void* host);
 /* run nsteps timesteps (asynchronous; replaces GpuBackend.run_step) */
 int onc_sim_step(onc_sim* sim, int nsteps);
 int onc_sim_sync(onc_sim* sim);

@@ -17,6 +17,8 @@ buffers, and has no loader. Here a bundle directory holds:
   cells.bin    the live cells as structure of arrays: column 0 for all cells,
                then column 1, ... (32-bit words, little-endian).
   fields.bin   each field's nx * ny * nz binary32 values, x fastest.
+  chem.bin     (when the model has a chemistry grid) each chemistry column's
+               32-bit words over the coarse grid, x fastest.
   rule.hip     the translated HIP source of the cell (and metric) rules,
                rebuilt with hiprtc on load.
 
@@ -59,6 +61,7 @@ def desc_to_dict(d: SimDesc) -> dict:
     out["fields"] = [{k: getattr(d.fields[f], k) for k in _FIELD_KEYS} for f in range(d.n_fields)]
     out["col_types"] = [int(d.col_types[c]) for c in range(d.n_cols)]
     out["params"] = [float(d.params[k]) for k in range(d.n_params)]
+    out["chem_col_types"] = [int(d.chem_col_types[c]) for c in range(d.n_chem_cols)]
     return out
 
 
@@ -86,6 +89,12 @@ def desc_from_dict(o: dict) -> SimDesc:
     d.n_params = len(ps)
     for k, v in enumerate(ps):
         d.params[k] = v
+    ch = o.get("chem_col_types", [])
+    if len(ch) > _abi.MAX_CHEM_COLS:
+        raise ValueError(f"a bundle holds at most {_abi.MAX_CHEM_COLS} chemistry columns")
+    d.n_chem_cols = len(ch)
+    for c, t in enumerate(ch):
+        d.chem_col_types[c] = int(t)
     return d
 
 
@@ -103,7 +112,8 @@ def get_config(sim: Sim, column_names: Optional[Sequence[str]] = None, kernel: O
         "execution_order": EXECUTION_ORDER,
         "module": None if kernel is None else {"path": "rule.hip", "kernel": kernel,
                                                "metric_fields": [list(m) for m in metric_fields]},
-        "buffers": {"cells": "cells.bin", "fields": "fields.bin"},
+        "buffers": {"cells": "cells.bin", "fields": "fields.bin",
+                    **({"chem": "chem.bin"} if sim.desc.n_chem_cols else {})},
     }
 
 
@@ -125,6 +135,10 @@ def export_bundle(sim: Sim, export_dir: str, column_names: Optional[Sequence[str
     with open(os.path.join(export_dir, "fields.bin"), "wb") as f:
         for k in range(sim.desc.n_fields):
             f.write(sim.download_field(k).astype("<f4").tobytes())
+    if sim.desc.n_chem_cols:
+        with open(os.path.join(export_dir, "chem.bin"), "wb") as f:
+            for k in range(sim.desc.n_chem_cols):
+                f.write(sim.download_chem(k).view(np.uint32).astype("<u4").tobytes())
     if module_src is not None:
         with open(os.path.join(export_dir, "rule.hip"), "w", encoding="utf-8") as f:
             f.write(module_src)
@@ -155,6 +169,12 @@ def load_bundle(ctx: _abi.Context, bundle: str) -> tuple[Sim, Optional[Module], 
     fld = np.fromfile(os.path.join(d, cfg["buffers"]["fields"]), dtype="<f4")
     if fld.size != nvox * desc.n_fields:
         raise ValueError("fields.bin does not hold one grid per field")
+    chem = None
+    if desc.n_chem_cols:
+        chem = np.fromfile(os.path.join(d, cfg["buffers"]["chem"]), dtype="<u4")
+        nchem = int(np.prod([max(1, int(v) // 2) for v in desc.dims[:3]]))
+        if chem.size != nchem * desc.n_chem_cols:
+            raise ValueError("chem.bin does not hold one grid per chemistry column")
     mod = None
     sim = Sim(ctx, desc)
     try:
@@ -169,6 +189,9 @@ def load_bundle(ctx: _abi.Context, bundle: str) -> tuple[Sim, Optional[Module], 
         sim.upload_cells(cols, next_id=int(st["next_id"]))
         for k in range(desc.n_fields):
             sim.upload_field(k, np.ascontiguousarray(fld[k * nvox:(k + 1) * nvox], dtype=np.float32))
+        for k in range(desc.n_chem_cols):
+            nchem = sim.nchem
+            sim.upload_chem(k, chem[k * nchem:(k + 1) * nchem].astype(np.uint32).view(dt[int(desc.chem_col_types[k])]))
         sim.set_step(int(st["step"]))
     except Exception:
         sim.close()

@@ -17,6 +17,7 @@ typedef unsigned long long onc_u64;
 #define ONC_DEV_MAX_COLS 48
 #define ONC_DEV_MAX_PARAMS 32
 #define ONC_DEV_MAX_PASSES 4
+#define ONC_DEV_MAX_CHEM 16
 #define ONC_HIST_COPIES 8  // copies of the per-sim digit histograms (atomic spread)
 
 // Device buffers reached through the segment table and the kernel arguments
@@ -105,6 +106,25 @@ struct onc_dsim {
     // carries; only occupied bins are written, so nothing is cleared per step.
     onc_u32 ONC_GLOBAL* bins;
     float ONC_GLOBAL* pos4;  // sorted positions as {x, y, z, 0}, written with the permute
+    // Coarse chemistry grid: cdims voxels per axis, n_chem 32-bit columns, two
+    // buffers. Every chemistry-rule pass reads one buffer and writes the other;
+    // with chem_passes passes per step, the buffer holding the state after s
+    // completed steps is (s * chem_passes) & 1.
+    int cdims[3];
+    int n_chem;
+    onc_u32 chem_passes;
+    onc_u32 pad_c;
+    onc_u32 ONC_GLOBAL* chem[2][ONC_DEV_MAX_CHEM];
+};
+
+// Arguments of the grid-rule kernels of a translated module (onc_tissue_pass,
+// onc_chem_pass): one launch per pass over every sim of the batch (blockIdx.y
+// = sim). A tissue pass j reads fields fld[par ^ 1 ^ (j & 1)] and writes
+// fld[par ^ (j & 1)]; a chemistry pass p of step s reads chem[(s P + p) & 1].
+struct onc_grid_args {
+    const onc_dsim ONC_CONST* sims;
+    onc_u32 nsims;
+    onc_u32 pass;
 };
 
 // Arguments of every batched kernel.
@@ -202,9 +222,10 @@ __device__ __forceinline__ void onc_philox(onc_u32 c0, onc_u32 c1, onc_u32 c2, o
 }
 
 // Per-cell counter-based stream: call c reads word c & 3 of
-// Philox(ctr = {id, step, c >> 2, 0}, key = seed).
+// Philox(ctr = {id, step, c >> 2, tag}, key = seed). Cells draw with tag 0;
+// voxels of grid-rule passes with a tag of their own (translated modules).
 struct onc_rng {
-    onc_u32 id, step, c, k0, k1;
+    onc_u32 id, step, c, k0, k1, tag;
     onc_u32 buf[4];
 };
 
@@ -212,12 +233,13 @@ __device__ __forceinline__ void onc_rng_init(onc_rng& r, const onc_dsim& S, onc_
     r.id = id;
     r.step = step;
     r.c = 0;
+    r.tag = 0;
     r.k0 = S.seed_lo;
     r.k1 = S.seed_hi;
 }
 
 __device__ __forceinline__ float onc_rand(onc_rng& r) {
-    if ((r.c & 3u) == 0) onc_philox(r.id, r.step, r.c >> 2, 0u, r.k0, r.k1, r.buf);
+    if ((r.c & 3u) == 0) onc_philox(r.id, r.step, r.c >> 2, r.tag, r.k0, r.k1, r.buf);
     // a select chain, not r.buf[r.c & 3]: a runtime index would put the
     // buffer in scratch memory
     const onc_u32 q = r.c & 3u;

@@ -92,5 +92,8 @@ hipError_t onc_launch_philox(const onc_u32* ctr, onc_u32 n, onc_u32 k0, onc_u32 
 hipError_t onc_launch_exscan(const onc_u32* in, onc_u32* out, onc_u32 n, onc_u32* total,
                              onc_u32* status, onc_u32* err, hipStream_t s);
 hipError_t onc_launch_fill_u32(onc_u32* p, onc_u32 v, size_t n, hipStream_t s);
+// fld[par] -> fld[par ^ 1] for every field of every unfrozen sim (after an odd
+// number of tissue-rule passes); gx blocks per sim
+hipError_t onc_launch_copy_fields(const onc_kargs& a, unsigned gx, hipStream_t s);
 hipError_t onc_launch_field_init(float* u, int X, int Y, int Z, const onc_dfield& f, hipStream_t s);
 hipError_t onc_launch_copy_u32(const onc_u32* src, onc_u32* dst, size_t n, hipStream_t s);

@@ -1819,6 +1819,25 @@ hipError_t onc_launch_exscan(const onc_u32* in, onc_u32* out, onc_u32 n, onc_u32
     }
     return hipGetLastError();
 }
+// After an odd number of tissue-rule passes the fields sit in fld[par]; the
+// cell rule and the commit expect them in fld[par ^ 1]. blockIdx.y = sim.
+__global__ __launch_bounds__(256) void k_copy_fields(onc_kargs a) {
+    const onc_dsim& S = a.sims[blockIdx.y];
+    if (S.frozen) return;
+    const size_t nvox = (size_t)S.dims[0] * (size_t)S.dims[1] * (size_t)S.dims[2];
+    const onc_u32 par = S.par;
+    for (int f = 0; f < S.n_fields; f++) {
+        const float* __restrict__ src = S.fld[par][f];
+        float* __restrict__ dst = S.fld[par ^ 1u][f];
+        for (size_t v = (size_t)blockIdx.x * 256u + threadIdx.x; v < nvox; v += (size_t)gridDim.x * 256u)
+            dst[v] = src[v];
+    }
+}
+hipError_t onc_launch_copy_fields(const onc_kargs& a, unsigned gx, hipStream_t s) {
+    if (a.nsims == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_fields, dim3(gx ? gx : 1, a.nsims), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 hipError_t onc_launch_fill_u32(onc_u32* p, onc_u32 v, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(256), 0, s, p, v, n);

@@ -99,6 +99,10 @@ struct onc_sim {
     int key_bits = 0;
     hipFunction_t rule_fn = nullptr;  // nullptr: built-in tumour rule
     onc_module* rule_mod = nullptr;
+    // translated grid rules of the rule module: passes per step and kernels
+    onc_u32 tissue_passes = 0, chem_passes = 0;
+    hipFunction_t tissue_fn = nullptr, chem_fn = nullptr;
+    size_t nchem = 0;  // chemistry voxels
     onc_batch* owner = nullptr;
     onc_batch* solo = nullptr;
 };
@@ -380,7 +384,22 @@ static int validate_desc(const onc_sim_desc* d) {
             return fail(ONC_ERR_INVALID, "bad field bc");
         if (!(fd.lo <= fd.hi)) return fail(ONC_ERR_INVALID, "field lo > hi");
     }
+    if (d->n_chem_cols < 0 || d->n_chem_cols > ONC_MAX_CHEM_COLS)
+        return fail(ONC_ERR_INVALID, "n_chem_cols must be in [0, 16]");
+    for (int c = 0; c < d->n_chem_cols; c++)
+        if (d->chem_col_types[c] < ONC_COL_F32 || d->chem_col_types[c] > ONC_COL_U32)
+            return fail(ONC_ERR_INVALID, "bad chemistry column type");
     return ONC_OK;
+}
+
+// Chemistry grid: max(1, dims / 2) voxels per axis.
+static void chem_dims(const onc_sim_desc* d, int cd[3]) {
+    for (int i = 0; i < 3; i++) cd[i] = std::max(1, d->dims[i] / 2);
+}
+static size_t chem_voxels(const onc_sim_desc* d) {
+    int cd[3];
+    chem_dims(d, cd);
+    return (size_t)cd[0] * (size_t)cd[1] * (size_t)cd[2];
 }
 
 static void sim_layout(const onc_sim_desc* d, uint32_t* table, int* key_bits, size_t* nvox, uint64_t* bytes) {
@@ -399,6 +418,7 @@ static void sim_layout(const onc_sim_desc* d, uint32_t* table, int* key_bits, si
     b += 16ull * cap;                 // packed sorted positions
     b += 2ull * d->n_fields * (*nvox) * 4;
     b += 4ull * (uint64_t)(*table) * 4;  // bins {start, end, epoch, 0}
+    b += 2ull * d->n_chem_cols * chem_voxels(d) * 4;  // chemistry grid, two buffers
     b += 256;
     *bytes = b;
 }
@@ -442,10 +462,14 @@ static void fill_dsim_static(onc_sim* s) {
     D.n_cols = d.n_cols;
     memcpy(D.fields, d.fields, sizeof(D.fields));
     memcpy(D.params, d.params, sizeof(D.params));
+    chem_dims(&d, D.cdims);
+    D.n_chem = d.n_chem_cols;
+    D.chem_passes = s->chem_passes;
 }
 
 static int batch_create_internal(onc_ctx* c, bool solo, onc_batch** out);
 static int batch_commit(onc_batch* b);
+static int sim_read_state(onc_sim* s, onc_dstate* st);
 
 extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) {
     if (!c || !out) return fail(ONC_ERR_INVALID, "NULL argument");
@@ -489,6 +513,14 @@ extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) 
     }
     if ((rc = sim_alloc(s, 4ull * s->table * 4, &p)) != ONC_OK) return cleanup(rc);
     s->dsim.bins = (onc_u32*)p;
+    s->nchem = chem_voxels(d);
+    void* chem_base = nullptr;
+    if (d->n_chem_cols > 0) {
+        if ((rc = sim_alloc(s, 2ull * d->n_chem_cols * s->nchem * 4, &chem_base)) != ONC_OK) return cleanup(rc);
+        for (int b = 0; b < 2; b++)
+            for (int c = 0; c < d->n_chem_cols; c++)
+                s->dsim.chem[b][c] = (onc_u32*)chem_base + ((size_t)b * d->n_chem_cols + c) * s->nchem;
+    }
     if ((rc = sim_alloc(s, sizeof(onc_dstate), &p)) != ONC_OK) return cleanup(rc);
     s->dstate = (onc_dstate*)p;
     s->dsim.state = s->dstate;
@@ -498,6 +530,8 @@ extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) 
     z.stop = 0xFFFFFFFFu;
     hipError_t e = hipMemcpyAsync(s->dstate, &z, sizeof(z), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = onc_launch_fill_u32(s->dsim.bins, 0xFFFFFFFFu, 4ull * s->table, st);
+    if (e == hipSuccess && chem_base)
+        e = onc_launch_fill_u32((onc_u32*)chem_base, 0u, 2ull * d->n_chem_cols * s->nchem, st);
     for (int f = 0; f < d->n_fields && e == hipSuccess; f++) {
         onc_dfield fd;
         memcpy(&fd, &d->fields[f], sizeof(fd));
@@ -534,20 +568,59 @@ extern "C" int onc_sim_destroy(onc_sim* s) {
     return ONC_OK;
 }
 
+// A u32 word of a module (0 when the module does not define it).
+static int module_word(onc_module* m, const char* name, onc_u32* out) {
+    hipDeviceptr_t gp = nullptr;
+    size_t bytes = 0;
+    *out = 0;
+    if (hipModuleGetGlobal(&gp, &bytes, m->mod, name) != hipSuccess) {
+        (void)hipGetLastError();
+        return ONC_OK;
+    }
+    if (bytes != sizeof(onc_u32)) return fail(ONC_ERR_INVALID, std::string("module word ") + name + " is not 32-bit");
+    CK(hipMemcpy(out, gp, sizeof(onc_u32), hipMemcpyDeviceToHost));
+    return ONC_OK;
+}
+
+// Copy the chemistry columns from buffer `from` to the other one.
+static int chem_copy(onc_sim* s, int from) {
+    for (int c = 0; c < s->desc.n_chem_cols; c++)
+        CK(hipMemcpy(s->dsim.chem[from ^ 1][c], s->dsim.chem[from][c], s->nchem * 4, hipMemcpyDeviceToDevice));
+    return ONC_OK;
+}
+
 extern "C" int onc_sim_set_rule(onc_sim* s, onc_module* m, const char* name) {
     if (!s) return fail(ONC_ERR_INVALID, "NULL sim");
-    if (!m) {
-        s->rule_fn = nullptr;
-        s->rule_mod = nullptr;
-    } else {
+    CK(hipSetDevice(s->ctx->device));
+    onc_u32 tp = 0, cp = 0;
+    hipFunction_t f = nullptr, tf = nullptr, cf = nullptr;
+    if (m) {
         if (!name) return fail(ONC_ERR_INVALID, "kernel name is NULL");
-        CK(hipSetDevice(s->ctx->device));
-        hipFunction_t f = nullptr;
         hipError_t e = hipModuleGetFunction(&f, m->mod, name);
         if (e != hipSuccess) return fail(ONC_ERR_INVALID, std::string("no kernel named ") + name + " in module");
-        s->rule_fn = f;
-        s->rule_mod = m;
+        CKR(module_word(m, "onc_grid_tissue_passes", &tp));
+        CKR(module_word(m, "onc_grid_chem_passes", &cp));
+        if (tp && hipModuleGetFunction(&tf, m->mod, "onc_tissue_pass") != hipSuccess)
+            return fail(ONC_ERR_INVALID, "module declares tissue passes but has no onc_tissue_pass kernel");
+        if (cp && hipModuleGetFunction(&cf, m->mod, "onc_chem_pass") != hipSuccess)
+            return fail(ONC_ERR_INVALID, "module declares chemistry passes but has no onc_chem_pass kernel");
+        if (tp && s->desc.n_fields == 0) return fail(ONC_ERR_INVALID, "tissue rules need tissue fields");
+        if (cp && s->desc.n_chem_cols == 0) return fail(ONC_ERR_INVALID, "chemistry rules need chemistry columns");
     }
+    if (cp != s->chem_passes && s->desc.n_chem_cols > 0) {
+        // the committed chemistry state stays where the new pass count expects it
+        onc_dstate st;
+        CKR(sim_read_state(s, &st));
+        const int cur_old = (int)((st.step * s->chem_passes) & 1u), cur_new = (int)((st.step * cp) & 1u);
+        if (cur_old != cur_new) CKR(chem_copy(s, cur_old));
+    }
+    s->rule_fn = f;
+    s->rule_mod = m;
+    s->tissue_passes = tp;
+    s->chem_passes = cp;
+    s->tissue_fn = tf;
+    s->chem_fn = cf;
+    s->dsim.chem_passes = cp;
     if (s->owner) s->owner->dirty = true;
     return ONC_OK;
 }
@@ -674,6 +747,30 @@ extern "C" int onc_sim_download_field(onc_sim* s, int f, float* host) {
     return ONC_OK;
 }
 
+static int chem_args(onc_sim* s, int col, const void* host, onc_u32** dev) {
+    if (!s || !host) return fail(ONC_ERR_INVALID, "NULL argument");
+    if (col < 0 || col >= s->desc.n_chem_cols) return fail(ONC_ERR_INVALID, "no such chemistry column");
+    CK(hipSetDevice(s->ctx->device));
+    onc_dstate st;
+    CKR(sim_read_state(s, &st));
+    *dev = s->dsim.chem[(st.step * s->chem_passes) & 1u][col];
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_upload_chem(onc_sim* s, int col, const void* host) {
+    onc_u32* dev = nullptr;
+    CKR(chem_args(s, col, host, &dev));
+    CK(hipMemcpy(dev, host, s->nchem * 4, hipMemcpyHostToDevice));
+    return ONC_OK;
+}
+
+extern "C" int onc_sim_download_chem(onc_sim* s, int col, void* host) {
+    onc_u32* dev = nullptr;
+    CKR(chem_args(s, col, host, &dev));
+    CK(hipMemcpy(host, dev, s->nchem * 4, hipMemcpyDeviceToHost));
+    return ONC_OK;
+}
+
 extern "C" int onc_sim_set_stop(onc_sim* s, uint32_t stop) {
     if (!s) return fail(ONC_ERR_INVALID, "NULL sim");
     CK(hipSetDevice(s->ctx->device));
@@ -698,6 +795,11 @@ extern "C" int onc_sim_set_step(onc_sim* s, uint32_t step) {
         hipError_t e = onc_launch_fill_u32(s->dsim.bins, 0xFFFFFFFFu, 4ull * s->table, s->ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
         if (e != hipSuccess) return fail(ONC_ERR_HIP, std::string("set_step: ") + hipGetErrorString(e));
+    }
+    if (s->desc.n_chem_cols > 0) {
+        // the committed chemistry state stays current under the new step count
+        const int cur_old = (int)((st.step * s->chem_passes) & 1u), cur_new = (int)((step * s->chem_passes) & 1u);
+        if (cur_old != cur_new) CKR(chem_copy(s, cur_old));
     }
     st.step = step;
     CK(hipMemcpy(s->dstate, &st, sizeof(st), hipMemcpyHostToDevice));
@@ -934,6 +1036,38 @@ static int grid_morton(onc_batch* b) {
     return (int)std::max<uint64_t>(1, std::min(need, lim));
 }
 
+// Translated tissue and chemistry rules (all sims of a batch share one
+// module): tissue passes, then a copy that leaves the fields in fld[par ^ 1]
+// when their count is odd, then the chemistry passes. One launch per pass for
+// the whole batch, blockIdx.y = sim.
+static int launch_grid_rules(onc_batch* b) {
+    if (b->sims.empty()) return ONC_OK;
+    const onc_sim* s0 = b->sims[0];
+    if (!s0->tissue_passes && !s0->chem_passes) return ONC_OK;
+    size_t nvox = 0, nchem = 0;
+    for (const onc_sim* s : b->sims) {
+        nvox = std::max(nvox, s->nvox);
+        nchem = std::max(nchem, s->nchem);
+    }
+    onc_grid_args g{};
+    g.sims = b->ka.sims;
+    g.nsims = (onc_u32)b->sims.size();
+    void* args[] = {&g};
+    const unsigned cap = (unsigned)b->ctx->cus * 4u;
+    const unsigned gx_t = (unsigned)std::min<size_t>((nvox + 255) / 256, cap);
+    const unsigned gx_c = (unsigned)std::min<size_t>((nchem + 255) / 256, cap);
+    for (onc_u32 j = 0; j < s0->tissue_passes; j++) {
+        g.pass = j;
+        CK(hipModuleLaunchKernel(s0->tissue_fn, gx_t, g.nsims, 1, 256, 1, 1, 0, b->stream, args, nullptr));
+    }
+    if (s0->tissue_passes & 1u) CK(onc_launch_copy_fields(b->ka, gx_t, b->stream));
+    for (onc_u32 p = 0; p < s0->chem_passes; p++) {
+        g.pass = p;
+        CK(hipModuleLaunchKernel(s0->chem_fn, gx_c, g.nsims, 1, 256, 1, 1, 0, b->stream, args, nullptr));
+    }
+    return ONC_OK;
+}
+
 static int launch_rule(onc_batch* b) {
     hipFunction_t fn = b->sims.empty() ? nullptr : b->sims[0]->rule_fn;
     if (!fn) {
@@ -1014,6 +1148,7 @@ static int enqueue_step(onc_batch* b) {
         CK(onc_launch_stencil(a, b->stencil_blocks, st));
         prof_mark(b, ONC_PH_STENCIL, 1);
     }
+    CKR(launch_grid_rules(b));
     if (b->rule_compact) {
         prof_mark(b, ONC_PH_RULE_COMPACT, 0);
         CK(onc_launch_rule_compact(a, b->citems, grid_ctiles(b), st));

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "liboncohip.so")
 MAX_FIELDS = 4
 MAX_COLS = 48
 MAX_PARAMS = 32
+MAX_CHEM_COLS = 16
 PHASES = 10
 
 OK = 0
@@ -67,6 +68,7 @@ class SimDesc(C.Structure):
         ("n_fields", C.c_int32), ("fields", FieldDesc * MAX_FIELDS),
         ("n_cols", C.c_int32), ("col_types", C.c_int32 * MAX_COLS),
         ("n_params", C.c_int32), ("params", C.c_float * MAX_PARAMS),
+        ("n_chem_cols", C.c_int32), ("chem_col_types", C.c_int32 * MAX_CHEM_COLS),
     ]
 
 
@@ -105,6 +107,8 @@ _PROTOS = {
     "onc_sim_download_cells": (C.c_int, [_P, C.POINTER(_P), _u32, C.POINTER(_u32)]),
     "onc_sim_upload_field": (C.c_int, [_P, C.c_int, _P]),
     "onc_sim_download_field": (C.c_int, [_P, C.c_int, _P]),
+    "onc_sim_upload_chem": (C.c_int, [_P, C.c_int, _P]),
+    "onc_sim_download_chem": (C.c_int, [_P, C.c_int, _P]),
     "onc_sim_step": (C.c_int, [_P, C.c_int]),
     "onc_sim_sync": (C.c_int, [_P]),
     "onc_sim_counts": (C.c_int, [_P, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),

@@ -29,12 +29,11 @@ from typing import Any, Optional
 import numpy as np
 
 from .. import _types as T
-from .._errors import CompilationError
 from . import _abi
 from ._abi import FieldDesc, SimDesc
 from ._sim import Module, Sim
-from ._translator import (COL_F32, COL_I32, COL_U32, F32, CellSchema, TissueSchema, Translator,
-                          param_values)
+from ._translator import (BOOL, COL_F32, COL_I32, COL_U32, F32, I32, U32, ChemSchema, CellSchema, TissueSchema,
+                          Translator, param_values)
 
 _CONTEXTS: dict[int, _abi.Context] = {}
 
@@ -59,6 +58,17 @@ def _close_contexts() -> None:
 
 
 _NP = {COL_F32: np.float32, COL_I32: np.int32, COL_U32: np.uint32}
+
+# 32-bit word type of a chemistry column of each field type
+_CHEM_NP = {F32: np.float32, I32: np.int32, U32: np.uint32, BOOL: np.uint32}
+
+
+def _default(cls, name, t):
+    v = getattr(cls, name, None)
+    if v is None or callable(v):
+        return 0
+    return float(v) if t == F32 else int(v)
+
 
 # no built-in forces unless the engine sets them (Engine.setup_mechanics)
 NO_MECHANICS = {"k_rep": 0.0, "k_adh": 0.0, "mobility": 0.25, "max_disp": 0.25, "interaction_radius": None,
@@ -129,6 +139,7 @@ class HipBackend:
         self.module: Optional[Module] = None
         self.schema: Optional[CellSchema] = None
         self.tissue: Optional[TissueSchema] = None
+        self.chem: Optional[ChemSchema] = None
         self.source: str = ""
         self._next_id = 0
         self.metric_fields: list = []  # (name, "f32" / "i32" / "u32") per metric word
@@ -143,16 +154,14 @@ class HipBackend:
     # ---- compile ---------------------------------------------------------------------
     def compile(self, sim_instance: Any) -> None:
         e = self.engine
-        if e._chemistry_rules:
-            raise CompilationError("chemistry rules are not supported by the HIP backend; declare the field on the "
-                                   "Tissue class with ol.diffusing(...) to have it diffused on the device")
-        if e._tissue_rules:
-            raise CompilationError("tissue rules are not supported by the HIP backend")
         spec = type(sim_instance)._spec
         self.schema = CellSchema.of(spec.cell_class)
         self.tissue = TissueSchema.of(spec.tissue_class if e.tissue is not None else None)
-        tr = Translator(sim_instance, self.schema, self.tissue, sim_instance._params)
-        self.source = tr.translate(e._cell_rules, e._metric_rules, spec.metrics_class)
+        has_chem = getattr(e, "chemistry", None) is not None or bool(e._chemistry_rules)
+        self.chem = ChemSchema.of(spec.chemistry_class if has_chem else None)
+        tr = Translator(sim_instance, self.schema, self.tissue, sim_instance._params, chem=self.chem)
+        self.source = tr.translate(e._cell_rules, e._metric_rules, spec.metrics_class,
+                                   tissue_rules=e._tissue_rules, chem_rules=e._chemistry_rules)
         self.metric_fields = list(tr.metric_fields)
         self.module = Module(self.ctx, self.source)
         self.sim = Sim(self.ctx, self._desc(sim_instance))
@@ -192,6 +201,10 @@ class HipBackend:
         d.n_cols = len(self.schema.columns)
         for k, col in enumerate(self.schema.columns):
             d.col_types[k] = col.coltype
+        ct = self.chem.col_types if self.chem is not None else []
+        d.n_chem_cols = len(ct)
+        for k, t in enumerate(ct):
+            d.chem_col_types[k] = t
         vals = param_values(sim_instance._params)
         d.n_params = len(vals)
         for k, v in enumerate(vals):
@@ -318,6 +331,16 @@ class HipBackend:
         if e.tissue is not None:
             for f, (name, t, _) in enumerate(self.tissue.fields):
                 self.sim.upload_field(f, _grid_field(e.tissue, name).astype(np.float32))
+        if self.chem is not None and self.chem.cls is not None:
+            grid = getattr(e, "chemistry", None)
+            for k, (name, t) in enumerate(self.chem.fields):
+                dt = _CHEM_NP[t]
+                if grid is not None:
+                    vals = np.asarray(_grid_field(grid, name))
+                    vals = vals.astype(np.float32) if t == F32 else vals.astype(np.int64).astype(dt)
+                else:
+                    vals = np.full(self.sim.nchem, _default(self.chem.cls, name, t), dtype=dt)
+                self.sim.upload_chem(k, vals)
 
     def download_columns(self) -> dict:
         """Cells as numpy columns keyed by field name ('pos' is (n, 3))."""
@@ -371,6 +394,11 @@ class HipBackend:
         if e.tissue is not None:
             for f, (name, t, _) in enumerate(self.tissue.fields):
                 _set_grid_field(e.tissue, name, self.sim.download_field(f))
+        grid = getattr(e, "chemistry", None)
+        if grid is not None and self.chem is not None and self.chem.cls is not None:
+            for k, (name, t) in enumerate(self.chem.fields):
+                v = self.sim.download_chem(k)
+                _set_grid_field(grid, name, v.astype(bool) if t == BOOL else v)
 
     def close(self) -> None:
         if self.sim is not None:

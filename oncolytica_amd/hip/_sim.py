@@ -47,6 +47,7 @@ class Sim:
         call("onc_sim_create", ctx.h, C.byref(desc), C.byref(h))
         self.h = h
         self.col_types = [desc.col_types[i] for i in range(desc.n_cols)]
+        self.chem_types = [desc.chem_col_types[i] for i in range(desc.n_chem_cols)]
         self.dims = tuple(desc.dims)
         self.module: Optional[Module] = None
 
@@ -110,6 +111,27 @@ class Sim:
     def download_field(self, f: int) -> np.ndarray:
         out = np.empty(self.nvox, dtype=np.float32)
         call("onc_sim_download_field", self.h, f, ptr(out))
+        return out
+
+    @property
+    def chem_dims(self) -> tuple[int, int, int]:
+        """Voxels per axis of the coarse chemistry grid: max(1, dims / 2)."""
+        return tuple(max(1, int(d) // 2) for d in self.dims)
+
+    @property
+    def nchem(self) -> int:
+        cx, cy, cz = self.chem_dims
+        return cx * cy * cz
+
+    def upload_chem(self, col: int, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=_DT[self.chem_types[col]]).reshape(-1)
+        if a.size != self.nchem:
+            raise ValueError("chemistry column size mismatch")
+        call("onc_sim_upload_chem", self.h, col, ptr(a))
+
+    def download_chem(self, col: int) -> np.ndarray:
+        out = np.empty(self.nchem, dtype=_DT[self.chem_types[col]])
+        call("onc_sim_download_chem", self.h, col, ptr(out))
         return out
 
     def set_stop(self, stop: int) -> None:

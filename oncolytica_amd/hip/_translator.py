@@ -1,4 +1,5 @@
-"""Python-AST -> HIP C++ translator for @ol.cell_rule methods.
+"""Python-AST -> HIP C++ translator for @ol.cell_rule, @ol.tissue_rule,
+@ol.chemistry_rule and @ol.metric_rule methods.
 
 Replaces the reference's AST -> WGSL translator
 (framework/src/oncolytica/gpu/compiler/_compiler.py:233-376,
@@ -48,7 +49,7 @@ CTYPE = {F32: "float", I32: "int", U32: "unsigned", BOOL: "bool", VEC3: "onc_v3"
 
 @dataclass(frozen=True)
 class StructT:
-    kind: str   # "cell" | "tissue" | "metrics"
+    kind: str   # "cell" | "tissue" | "chem" | "metrics"
     cls: type
 
     @property
@@ -99,6 +100,8 @@ def py_to_dsl(ann) -> Any:
         return StructT("cell", ann)
     if isinstance(ann, type) and issubclass(ann, T.Tissue):
         return StructT("tissue", ann)
+    if isinstance(ann, type) and issubclass(ann, T.Chemistry):
+        return StructT("chem", ann)
     origin = typing.get_origin(ann)
     if origin is tuple:
         return TupleT(tuple(py_to_dsl(a) for a in typing.get_args(ann)))
@@ -185,6 +188,32 @@ class TissueSchema:
         if len(out) > 4:
             raise CompilationError("a tissue may hold at most 4 fields on the HIP backend")
         return cls_(tissue_cls, out)
+
+
+@dataclass
+class ChemSchema:
+    """The Chemistry record of the coarse grid: scalar fields, one 32-bit
+    column each (f32, i32, u32; bool as u32)."""
+    cls: Optional[type]
+    fields: list      # (name, dsl type)
+
+    @classmethod
+    def of(cls_, chem_cls) -> "ChemSchema":
+        if chem_cls is None:
+            return cls_(None, [])
+        out = []
+        for n, t in data_fields(chem_cls):
+            dt = py_to_dsl(t)
+            if dt not in SCALARS:
+                raise CompilationError(f"Chemistry field '{n}': only scalar fields are supported on the HIP backend")
+            out.append((n, dt))
+        if len(out) > 16:
+            raise CompilationError("a chemistry record may hold at most 16 fields on the HIP backend")
+        return cls_(chem_cls, out)
+
+    @property
+    def col_types(self) -> list[int]:
+        return [COL_F32 if t == F32 else (COL_I32 if t == I32 else COL_U32) for _, t in self.fields]
 
 
 @dataclass
@@ -340,11 +369,19 @@ class Translator:
     KERNEL = "onc_rule_main"
     METRIC_KERNELS = ("onc_metric_main", "onc_metric_final")
 
-    def __init__(self, sim, cell_schema: CellSchema, tissue: TissueSchema, params):
+    TISSUE_KERNEL = "onc_tissue_pass"
+    CHEM_KERNEL = "onc_chem_pass"
+
+    def __init__(self, sim, cell_schema: CellSchema, tissue: TissueSchema, params,
+                 chem: Optional[ChemSchema] = None):
         self.sim = sim
         self.sim_cls = type(sim)
         self.cell = cell_schema
         self.tissue = tissue
+        self.chem = chem if chem is not None else ChemSchema(None, [])
+        self.rule_kind = "cell"  # kind of the rule being translated: cell, tissue, chem, metric
+        self.tissue_passes = 0
+        self.chem_pass_rules: list[str] = []  # rule function of each chemistry pass of a step
         self.pslots = {p.name: p for p in param_slots(params)}
         self.structs: dict[str, StructT] = {}
         self.tuples: dict[str, TupleT] = {}
@@ -358,10 +395,42 @@ class Translator:
         self.metric_fields: list[tuple[str, str]] = []  # (name, f32 / i32 / u32), in word order
 
     # ---- entry ---------------------------------------------------------------------
-    def translate(self, rules: list, metric_rules: list = (), metrics_cls=None) -> str:
+    def translate(self, rules: list, metric_rules: list = (), metrics_cls=None, tissue_rules: list = (),
+                  chem_rules: list = ()) -> str:
         self.struct_of(StructT("cell", self.cell.cls))
         if self.tissue.cls is not None:
             self.struct_of(StructT("tissue", self.tissue.cls))
+        if self.chem.cls is not None:
+            self.struct_of(StructT("chem", self.chem.cls))
+        tissue_names, chem_names = [], []
+        for kind, lst, out in (("tissue", tissue_rules, tissue_names), ("chem", chem_rules, chem_names)):
+            for r in lst:
+                fn = r.__func__ if hasattr(r, "__func__") else r
+                node = self._parse(fn)
+                args = node.args.args
+                rec = self.tissue.cls if kind == "tissue" else self.chem.cls
+                what = "tissue" if kind == "tissue" else "chemistry"
+                if rec is None:
+                    raise CompilationError(f"{what} rule '{fn.__name__}' needs the engine's {what} grid", node)
+                if len(args) != 2:
+                    raise CompilationError(f"{what} rule '{fn.__name__}' must take (self, voxel)", node)
+                key = f"{kind}rule:{fn.__name__}"
+                info = FuncInfo(key, f"{kind}rule_{fn.__name__}", node, _namespace(fn), "sim", VOID,
+                                [(args[1].arg, StructT(kind, rec))])
+                self.funcs[key] = info
+                self.rule_kind = kind
+                try:
+                    self._emit_func(info, is_rule=True)
+                finally:
+                    self.rule_kind = "cell"
+                out.append(info.cname)
+                if kind == "chem":
+                    iters = int(getattr(fn, "_iterations", 1))
+                    if iters < 0:
+                        raise CompilationError(f"chemistry rule '{fn.__name__}': iterations must be >= 0", node)
+                    self.chem_pass_rules += [info.cname] * iters
+        self.tissue_passes = len(tissue_names)
+        self.grid_rules = (tissue_names, chem_names)
         rule_names = []
         for r in rules:
             fn = r.__func__ if hasattr(r, "__func__") else r
@@ -396,10 +465,12 @@ class Translator:
                             [(args[1].arg, StructT("cell", self.cell.cls)), (args[2].arg, mt)])
             self.funcs[key] = info
             self.in_metric = True
+            self.rule_kind = "metric"
             try:
                 self._emit_func(info, is_rule=True)
             finally:
                 self.in_metric = False
+                self.rule_kind = "cell"
             metric_names.append(info.cname)
         return self._assemble(rule_names, metric_names)
 
@@ -433,6 +504,8 @@ class Translator:
             return out
         if st.kind == "cell":
             return list(self.cell.fields) if st.cls is self.cell.cls else [(n, py_to_dsl(t)) for n, t in data_fields(st.cls)]
+        if st.kind == "chem":
+            return list(self.chem.fields)
         return [(n, t) for n, t, _ in self.tissue.fields]
 
     def field_type(self, st: StructT, name: str, node):
@@ -458,8 +531,7 @@ class Translator:
         env = Env()
         sig = []
         if info.self_t != "sim":
-            const = "const " if info.self_t.kind == "tissue" else ""
-            sig.append(f"{const}{ctype(info.self_t)}& self")
+            sig.append(f"{ctype(info.self_t)}& self")
             env.vars[info.node.args.args[0].arg] = Local("self", info.self_t)
         for pname, pt in info.params:
             ref = "&" if (is_rule and isinstance(pt, StructT)) else ""
@@ -513,8 +585,8 @@ class Translator:
                 yield from self._walk_stmts(st.body)
                 yield from self._walk_stmts(st.orelse)
             elif isinstance(st, ast.For):
-                if self._is_neighbors(st.iter):
-                    continue  # the loop body has its own scope (the neighbour variable)
+                if self._is_macro_loop(st.iter):
+                    continue  # the loop body has its own scope (the loop variable)
                 yield from self._walk_stmts(st.body)
 
     def zero(self, t) -> str:
@@ -705,8 +777,13 @@ class Translator:
                 raise CompilationError("metric fields take += and -= only", node)
             bc, bt = self.expr(node.value, env)
             if isinstance(bt, StructT):
-                if bt.kind == "tissue":
-                    raise CompilationError("cell rules may not write tissue fields", node)
+                if bt.kind in ("tissue", "chem"):
+                    what = "tissue" if bt.kind == "tissue" else "chemistry"
+                    if self.rule_kind in ("cell", "metric") and self.cur.key.split(":")[0] in ("rule", "metric"):
+                        raise CompilationError(f"cell rules may not write {what} fields", node)
+                    if bc.startswith("_g"):
+                        raise CompilationError(f"neighbouring {what} voxels are read-only", node)
+                    return f"{bc}.{node.attr}", self.field_type(bt, node.attr, node)
                 if self.in_metric and bt.kind == "cell":
                     raise CompilationError("metric rules read cells; they may not write them", node)
                 if bc.startswith("_nb"):
@@ -742,11 +819,24 @@ class Translator:
     def _is_neighbors(self, it) -> bool:
         return isinstance(it, ast.Attribute) and it.attr == "neighbors"
 
+    def _is_macro_loop(self, it) -> bool:
+        return isinstance(it, ast.Attribute) and it.attr in ("neighbors", "cells", "tissues")
+
     def st_For(self, node, env):
         if node.orelse:
             raise CompilationError("for ... else is not supported", node)
-        if self._is_neighbors(node.iter):
-            return self._for_neighbors(node, env)
+        if self._is_macro_loop(node.iter):
+            st = self.expr_type(node.iter.value, env)
+            kind = st.kind if isinstance(st, StructT) else None
+            if node.iter.attr == "neighbors" and kind == "cell":
+                return self._for_neighbors(node, env)
+            if node.iter.attr == "neighbors" and kind in ("tissue", "chem"):
+                return self._for_grid(node, env, "neighbors", st)
+            if node.iter.attr == "cells" and kind in ("tissue", "chem"):
+                return self._for_grid(node, env, "cells", st)
+            if node.iter.attr == "tissues" and kind == "chem":
+                return self._for_grid(node, env, "tissues", st)
+            raise CompilationError(f"'.{node.iter.attr}' is not defined on this value", node)
         it = node.iter
         if isinstance(it, ast.Call) and isinstance(it.func, ast.Name) and it.func.id == "range":
             args = [self.conv(*self.expr(a, env), I32, a) for a in it.args]
@@ -827,6 +917,87 @@ class Translator:
         self.emit("}")
         self.indent -= 1
         self.emit("}")
+        self.indent -= 1
+        self.emit(f"}} {end}:;")
+
+    def _for_grid(self, node, env, what, st):
+        """Voxel macros (_stmt_translator.py:493-608, restated):
+        `.neighbors` of a tissue or chemistry voxel: its 26 Moore neighbours in
+        template.wgsl:11-21 order (x fastest), out-of-range ones skipped, read
+        from the pass's input buffer; `.cells` of a tissue voxel: the cells of
+        its bin, and of a chemistry voxel: those of its 8 tissue voxels, read
+        as the cell rules' neighbours are (this step's sorted state);
+        `.tissues` of a chemistry voxel: its 8 tissue voxels (dz, dy, dx
+        order, x fastest, clipped at the grid, core/utils/_types.py:508-524)."""
+        if self.in_metric:
+            raise CompilationError("metric rules may not walk voxels", node)
+        if not isinstance(node.target, ast.Name):
+            raise CompilationError("voxel loop needs a simple variable", node)
+        sc, _ = self.expr(node.iter.value, env)
+        var, end, sv = self.fresh("gv"), self.fresh("gend"), self.fresh("gsrc")
+        self.emit("{")
+        self.indent += 1
+        self.emit(f"const {ctype(st)}& {sv} = {sc};")
+        self.emit("const onc_dsim& _S = *ctx.S;")
+        chem = st.kind == "chem"
+        dims = "_S.cdims" if chem else "_S.dims"
+        if what == "neighbors":
+            vt = st
+            self.emit(f"for (int _k = 0; _k < 27; _k++) {{")
+            self.indent += 1
+            self.emit("if (_k == 13) continue;")
+            self.emit(f"const int _nx = {sv}._coord.x + _k % 3 - 1, _ny = {sv}._coord.y + (_k / 3) % 3 - 1, "
+                      f"_nz = {sv}._coord.z + _k / 9 - 1;")
+            self.emit(f"if (_nx < 0 || _ny < 0 || _nz < 0 || _nx >= {dims}[0] || _ny >= {dims}[1] || "
+                      f"_nz >= {dims}[2]) continue;")
+            buf = "ctx.cin" if chem else "ctx.tin"
+            self.emit(f"{ctype(vt)} {var} = {vt.cname}_loadv(ctx, {buf}, _nx, _ny, _nz);")
+            closers = 1
+        elif what == "tissues":
+            if self.tissue.cls is None:
+                raise CompilationError("'.tissues' needs the engine's tissue grid", node)
+            vt = self.struct_of(StructT("tissue", self.tissue.cls))
+            self.emit("for (int _k = 0; _k < 8; _k++) {")
+            self.indent += 1
+            self.emit(f"const int _nx = 2 * {sv}._coord.x + (_k & 1), _ny = 2 * {sv}._coord.y + ((_k >> 1) & 1), "
+                      f"_nz = 2 * {sv}._coord.z + (_k >> 2);")
+            self.emit("if (_nx >= _S.dims[0] || _ny >= _S.dims[1] || _nz >= _S.dims[2]) continue;")
+            self.emit(f"{ctype(vt)} {var} = {vt.cname}_loadv(ctx, ctx.tin, _nx, _ny, _nz);")
+            closers = 1
+        else:  # cells
+            vt = self.struct_of(StructT("cell", self.cell.cls))
+            if chem:
+                self.emit("for (int _k = 0; _k < 8; _k++) {")
+                self.indent += 1
+                self.emit(f"const int _nx = 2 * {sv}._coord.x + (_k & 1), _ny = 2 * {sv}._coord.y + ((_k >> 1) & 1), "
+                          f"_nz = 2 * {sv}._coord.z + (_k >> 2);")
+                self.emit("if (_nx >= _S.dims[0] || _ny >= _S.dims[1] || _nz >= _S.dims[2]) continue;")
+                closers = 1
+            else:
+                self.emit(f"const int _nx = {sv}._coord.x, _ny = {sv}._coord.y, _nz = {sv}._coord.z;")
+                self.emit("{")
+                self.indent += 1
+                closers = 1
+            self.emit("const uint4 _b = ((const uint4*)_S.bins)[onc_morton((onc_u32)_nx, (onc_u32)_ny, "
+                      "(onc_u32)_nz, _S.key_format)];")
+            self.emit("if (_b.z != ctx.epoch) continue;" if chem else "if (_b.z == ctx.epoch)")
+            j = self.fresh("j")
+            self.emit(f"for (onc_u32 {j} = _b.x; {j} < _b.y; {j}++) {{")
+            self.indent += 1
+            self.emit(f"{ctype(vt)} {var} = {vt.cname}_load(_S, _S.st[ctx.par ^ 1u], {j});")
+            closers += 1
+        inner = Env(env)
+        inner.vars[node.target.id] = Local(var, vt)
+        saved = (getattr(self, "_nb_break", None), getattr(self, "_loop_depth_at_nb", None))
+        self._loop_depth = getattr(self, "_loop_depth", 0) + 1
+        self._nb_break, self._loop_depth_at_nb = end, self._loop_depth
+        self._declare_locals(node.body, inner)
+        self.stmts(node.body, inner)
+        self._nb_break, self._loop_depth_at_nb = saved
+        self._loop_depth -= 1
+        for _ in range(closers):
+            self.indent -= 1
+            self.emit("}")
         self.indent -= 1
         self.emit(f"}} {end}:;")
 
@@ -949,6 +1120,14 @@ class Translator:
         if isinstance(bt, StructT):
             if bt.kind == "metrics":
                 raise CompilationError("metric fields can only be added to inside a metric rule", node)
+            if bt.kind in ("tissue", "chem") and node.attr in ("coord", "pos") and \
+                    node.attr not in [n for n, _ in self.struct_fields(bt)]:
+                if node.attr == "coord":
+                    return f"{bc}._coord", IVEC3
+                # voxel centre: coord * size + size / 2 (core/utils/_types.py:465-467, 496-498)
+                vs = "(2.0f * ctx.S->dx)" if bt.kind == "chem" else "ctx.S->dx"
+                return (f"onc_v3_make((float){bc}._coord.x * {vs} + {vs} / 2.0f, (float){bc}._coord.y * {vs} + "
+                        f"{vs} / 2.0f, (float){bc}._coord.z * {vs} + {vs} / 2.0f)"), VEC3
             return f"{bc}.{node.attr}", self.field_type(bt, node.attr, node)
         if bt in (VEC3, IVEC3) and node.attr in ("x", "y", "z"):
             return f"{bc}.{node.attr}", F32 if bt == VEC3 else I32
@@ -1054,8 +1233,13 @@ class Translator:
                 st = StructT("tissue", self.tissue.cls)
                 return f"{st.cname}_at(ctx, {self.conv(pc, pt, VEC3, node)})", st
             if f.attr == "chemistry_at":
-                raise CompilationError("chemistry_at() is not supported by the HIP backend; declare a diffusing "
-                                       "Tissue field with ol.diffusing(...) instead", node)
+                if self.chem.cls is None:
+                    raise CompilationError("chemistry_at() needs engine.setup_chemistry(...)", node)
+                if self.in_metric:
+                    raise CompilationError("metric rules may not read the chemistry grid", node)
+                pc, pt = self.expr(node.args[0], env)
+                st = self.struct_of(StructT("chem", self.chem.cls))
+                return f"{st.cname}_at(ctx, {self.conv(pc, pt, VEC3, node)})", st
             meth = getattr(self.sim_cls, f.attr, None)
             if meth is None or not callable(meth):
                 raise CompilationError(f"simulation has no method '{f.attr}'", node)
@@ -1193,6 +1377,9 @@ class Translator:
         if st.kind == "cell":
             out.append("    onc_u32 _id;")
             out.append("    onc_u32 _idx;")
+        if st.kind in ("tissue", "chem"):
+            out.append("    onc_iv3 _coord;")
+            out.append("    onc_u32 _v;")
         return out
 
     def _defaults_expr(self, st: StructT) -> list[str]:
@@ -1226,6 +1413,8 @@ class Translator:
         out.append(f"""struct RuleCtx {{
     const onc_dsim* S;
     onc_u32 par, i, epoch;
+    onc_u32 tin;   // tissue field buffer read by tissue_at / voxel loads
+    onc_u32 cin;   // chemistry buffer read by chemistry_at / voxel loads
     bool alive, div;
     onc_rng rng;
     {cs.cname} daughter;
@@ -1260,12 +1449,48 @@ class Translator:
                    + "\n".join(stl) + "\n}\n")
         if self.tissue.cls is not None:
             ts = StructT("tissue", self.tissue.cls)
-            tl = []
+            tl, tw = [], []
             for f, (n, t, _) in enumerate(self.tissue.fields):
-                tl.append(f"    r.{n} = {self.conv(f'S.fld[ctx.par ^ 1u][{f}][v]', F32, t)};")
+                tl.append(f"    r.{n} = {self.conv(f'S.fld[buf][{f}][r._v]', F32, t)};")
+                tw.append(f"    S.fld[buf][{f}][r._v] = {self.conv(f'r.{n}', t, F32)};")
+            out.append(f"__device__ __forceinline__ {ts.cname} {ts.cname}_loadv(RuleCtx& ctx, onc_u32 buf, int x, int y, int z) {{\n"
+                       f"    const onc_dsim& S = *ctx.S;\n    {ts.cname} r;\n    r._coord = onc_iv3_make(x, y, z);\n"
+                       f"    r._v = (onc_u32)x + (onc_u32)S.dims[0] * ((onc_u32)y + (onc_u32)S.dims[1] * (onc_u32)z);\n"
+                       + "\n".join(tl) + "\n    return r;\n}\n")
+            out.append(f"__device__ __forceinline__ void {ts.cname}_storev(RuleCtx& ctx, onc_u32 buf, const {ts.cname}& r) {{\n"
+                       f"    const onc_dsim& S = *ctx.S;\n" + "\n".join(tw) + "\n}\n")
             out.append(f"__device__ __forceinline__ {ts.cname} {ts.cname}_at(RuleCtx& ctx, onc_v3 p) {{\n"
-                       f"    const onc_dsim& S = *ctx.S;\n    const onc_u32 v = onc_field_index(S, p.x, p.y, p.z);\n"
-                       f"    {ts.cname} r;\n" + "\n".join(tl) + "\n    return r;\n}\n")
+                       f"    const onc_dsim& S = *ctx.S;\n"
+                       f"    return {ts.cname}_loadv(ctx, ctx.tin, onc_vox1(p.x, S.dx, S.dims[0]), "
+                       f"onc_vox1(p.y, S.dx, S.dims[1]), onc_vox1(p.z, S.dx, S.dims[2]));\n}}\n")
+        if self.chem.cls is not None:
+            ch = StructT("chem", self.chem.cls)
+            cl, cw = [], []
+            for k, (n, t) in enumerate(self.chem.fields):
+                w = f"S.chem[buf][{k}][r._v]"
+                if t == F32:
+                    cl.append(f"    r.{n} = __uint_as_float({w});")
+                    cw.append(f"    {w} = __float_as_uint(r.{n});")
+                elif t == I32:
+                    cl.append(f"    r.{n} = (int){w};")
+                    cw.append(f"    {w} = (onc_u32)r.{n};")
+                elif t == BOOL:
+                    cl.append(f"    r.{n} = {w} != 0u;")
+                    cw.append(f"    {w} = r.{n} ? 1u : 0u;")
+                else:
+                    cl.append(f"    r.{n} = {w};")
+                    cw.append(f"    {w} = r.{n};")
+            out.append(f"__device__ __forceinline__ {ch.cname} {ch.cname}_loadv(RuleCtx& ctx, onc_u32 buf, int x, int y, int z) {{\n"
+                       f"    const onc_dsim& S = *ctx.S;\n    {ch.cname} r;\n    r._coord = onc_iv3_make(x, y, z);\n"
+                       f"    r._v = (onc_u32)x + (onc_u32)S.cdims[0] * ((onc_u32)y + (onc_u32)S.cdims[1] * (onc_u32)z);\n"
+                       + "\n".join(cl) + "\n    return r;\n}\n")
+            out.append(f"__device__ __forceinline__ void {ch.cname}_storev(RuleCtx& ctx, onc_u32 buf, const {ch.cname}& r) {{\n"
+                       f"    const onc_dsim& S = *ctx.S;\n" + "\n".join(cw) + "\n}\n")
+            # voxel of pos: floor(pos / (2 dx)) clamped (core/utils/_types.py:332-346)
+            out.append(f"__device__ __forceinline__ {ch.cname} {ch.cname}_at(RuleCtx& ctx, onc_v3 p) {{\n"
+                       f"    const onc_dsim& S = *ctx.S;\n    const float cdx = 2.0f * S.dx;\n"
+                       f"    return {ch.cname}_loadv(ctx, ctx.cin, onc_vox1(p.x, cdx, S.cdims[0]), "
+                       f"onc_vox1(p.y, cdx, S.cdims[1]), onc_vox1(p.z, cdx, S.cdims[2]));\n}}\n")
         for key in self.order:
             out.append(self.funcs[key].proto)
         for key in self.order:
@@ -1279,6 +1504,8 @@ class Translator:
         ctx.par = S.par;
         ctx.i = i;
         ctx.epoch = S.step + 1u;
+        ctx.tin = ctx.par ^ 1u;
+        ctx.cin = (S.step * S.chem_passes + S.chem_passes) & 1u;
         ctx.alive = true;
         ctx.div = false;
         {cs.cname} cell = {cs.cname}_load(S, S.st[ctx.par ^ 1u], i);
@@ -1293,8 +1520,85 @@ class Translator:
     }});
 }}
 """)
+        out.append(self._grid_kernels())
         if metric_names:
             out.append(self._metric_kernels(metric_names))
+        return "\n".join(out)
+
+    def _grid_kernels(self) -> str:
+        """onc_tissue_pass / onc_chem_pass and the pass counts the runtime
+        reads (onc_sim_set_rule). Tissue pass j runs tissue rule j; chemistry
+        passes run each chemistry rule `iterations` times in declaration order
+        (gpu/runtime/_gpu_backend.py:127-142). Every pass is Jacobi: voxels read
+        one buffer and write the other. Random draws of voxel v: Philox
+        ctr = {v, step, call >> 2, tag}, tag 0x10000 + j (tissue pass j) or
+        0x20000 + p (chemistry pass p); cells draw with tag 0."""
+        tissue_names, chem_names = getattr(self, "grid_rules", ([], []))
+        passes = self.chem_pass_rules
+        out = [f'extern "C" __device__ onc_u32 onc_grid_tissue_passes = {len(tissue_names)}u;',
+               f'extern "C" __device__ onc_u32 onc_grid_chem_passes = {len(passes)}u;', ""]
+        if tissue_names:
+            ts = StructT("tissue", self.tissue.cls)
+            cases = "\n".join(f"            case {j}u: {n}(t, ctx); break;" for j, n in enumerate(tissue_names))
+            out.append(f"""extern "C" __global__ __launch_bounds__(256) void {self.TISSUE_KERNEL}(onc_grid_args a) {{
+    const onc_dsim& S = a.sims[blockIdx.y];
+    if (S.frozen) return;
+    const onc_u32 X = (onc_u32)S.dims[0], Y = (onc_u32)S.dims[1], Z = (onc_u32)S.dims[2];
+    const onc_u32 nvox = X * Y * Z;
+    RuleCtx ctx;
+    ctx.S = &S;
+    ctx.par = S.par;
+    ctx.i = 0xFFFFFFFFu;
+    ctx.epoch = S.step + 1u;
+    ctx.tin = S.par ^ 1u ^ (a.pass & 1u);
+    ctx.cin = (S.step * S.chem_passes) & 1u;
+    ctx.alive = true;
+    ctx.div = false;
+    const onc_u32 tout = S.par ^ (a.pass & 1u);
+    for (onc_u32 v = blockIdx.x * 256u + threadIdx.x; v < nvox; v += gridDim.x * 256u) {{
+        const onc_u32 x = v % X, y = (v / X) % Y, z = v / (X * Y);
+        {ts.cname} t = {ts.cname}_loadv(ctx, ctx.tin, (int)x, (int)y, (int)z);
+        onc_rng_init(ctx.rng, S, v, S.step);
+        ctx.rng.tag = 0x10000u + a.pass;
+        switch (a.pass) {{
+{cases}
+            default: break;
+        }}
+        {ts.cname}_storev(ctx, tout, t);
+    }}
+}}
+""")
+        if passes:
+            chs = StructT("chem", self.chem.cls)
+            cases = "\n".join(f"            case {p}u: {n}(c, ctx); break;" for p, n in enumerate(passes))
+            out.append(f"""extern "C" __global__ __launch_bounds__(256) void {self.CHEM_KERNEL}(onc_grid_args a) {{
+    const onc_dsim& S = a.sims[blockIdx.y];
+    if (S.frozen) return;
+    const onc_u32 X = (onc_u32)S.cdims[0], Y = (onc_u32)S.cdims[1], Z = (onc_u32)S.cdims[2];
+    const onc_u32 nvox = X * Y * Z;
+    RuleCtx ctx;
+    ctx.S = &S;
+    ctx.par = S.par;
+    ctx.i = 0xFFFFFFFFu;
+    ctx.epoch = S.step + 1u;
+    ctx.tin = S.par ^ 1u;
+    ctx.cin = (S.step * S.chem_passes + a.pass) & 1u;
+    ctx.alive = true;
+    ctx.div = false;
+    const onc_u32 cout = ctx.cin ^ 1u;
+    for (onc_u32 v = blockIdx.x * 256u + threadIdx.x; v < nvox; v += gridDim.x * 256u) {{
+        const onc_u32 x = v % X, y = (v / X) % Y, z = v / (X * Y);
+        {chs.cname} c = {chs.cname}_loadv(ctx, ctx.cin, (int)x, (int)y, (int)z);
+        onc_rng_init(ctx.rng, S, v, S.step);
+        ctx.rng.tag = 0x20000u + a.pass;
+        switch (a.pass) {{
+{cases}
+            default: break;
+        }}
+        {chs.cname}_storev(ctx, cout, c);
+    }}
+}}
+""")
         return "\n".join(out)
 
     def _metric_kernels(self, metric_names: list[str]) -> str:
@@ -1354,6 +1658,8 @@ extern "C" __global__ __launch_bounds__(256) void onc_metric_main(onc_metric_arg
         ctx.par = a.parity ^ 1u;
         ctx.i = i;
         ctx.epoch = 0u;
+        ctx.tin = a.parity;
+        ctx.cin = 0u;
         ctx.alive = true;
         ctx.div = false;
         {cs.cname} cell = i < a.split ? {cs.cname}_load(S, S.st[a.parity], i) : {cs.cname}_load(S, S.dau2, i - a.split);

@@ -494,3 +494,70 @@ int FN(oracle_tumor_run)(const onc_sim_desc* d, uint32_t* n_io, uint32_t* next_i
     for (int f = 0; f < d->n_fields; f++) free(swept[f]);
     return 0;
 }
+
+/* ---- steps 1-6 for any rule ------------------------------------------------
+ * The part of the timestep before the rules (the same code the tumour step
+ * above runs): keys of the n cells and their stable sort (perm[i] = index of
+ * the cell at sorted position i, keys[i] its key), uptake, forces with the
+ * integration, and one sweep of every field. spx..spz receive the sorted
+ * positions before the forces, ipx..ipz the integrated ones (sorted order);
+ * fields are replaced by their swept values. Arrays hold n elements. */
+int FN(oracle_pre_rule)(const onc_sim_desc* d, uint32_t n, const REAL* px, const REAL* py, const REAL* pz,
+                        uint32_t* perm, uint32_t* keys, REAL* spx, REAL* spy, REAL* spz, REAL* ipx, REAL* ipy,
+                        REAL* ipz, REAL* const* fields) {
+    size_t nvox = (size_t)d->dims[0] * (size_t)d->dims[1] * (size_t)d->dims[2];
+    REAL dx = (REAL)d->dx;
+    FN(oracle_keys)(px, py, pz, n, d->dims, dx, d->key_format, keys);
+    for (uint32_t i = 0; i < n; i++) perm[i] = i;
+    oracle_radix_sort_pairs(keys, perm, n, oracle_key_bits(d->dims));
+    for (uint32_t i = 0; i < n; i++) {
+        spx[i] = px[perm[i]];
+        spy[i] = py[perm[i]];
+        spz[i] = pz[perm[i]];
+    }
+    REAL* t[3] = {spx, spy, spz};
+    uptake(d, keys, n, t, fields);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (uint32_t i = 0; i < n; i++) {
+        REAL Fx, Fy, Fz;
+        force_on(d, keys, n, spx, spy, spz, i, &Fx, &Fy, &Fz);
+        REAL x = spx[i], y = spy[i], z = spz[i];
+        integrate(d, Fx, Fy, Fz, &x, &y, &z);
+        ipx[i] = x;
+        ipy[i] = y;
+        ipz[i] = z;
+    }
+    for (int f = 0; f < d->n_fields; f++) {
+        REAL* sw = (REAL*)malloc((nvox ? nvox : 1) * sizeof(REAL));
+        if (!sw) return -1;
+        FN(oracle_stencil7)(fields[f], sw, d->dims, &d->fields[f]);
+        memcpy(fields[f], sw, nvox * sizeof(REAL));
+        free(sw);
+    }
+    return 0;
+}
+
+/* Unit vector from two uniforms (the operation sequence of rng_dir above),
+ * for draws of streams with another counter layout. */
+void FN(oracle_dir_from_uniforms)(REAL u1, REAL u2, REAL out[3]) {
+    REAL zz = u1 * (REAL)2 - (REAL)1;
+    REAL rr = (REAL)1 - zz * zz;
+    if (rr < (REAL)0) rr = (REAL)0;
+    REAL rad = RSQRT(rr);
+    REAL t = u2 * (REAL)4;
+    int32_t q = (int32_t)t;
+    REAL f = t - (REAL)q;
+    REAL a = f * (REAL)1.5707963267948966;
+    REAL s, c;
+    FN(oracle_sincos_q)(a, &s, &c);
+    REAL cx, sy;
+    switch (q & 3) {
+        case 0: cx = c; sy = s; break;
+        case 1: cx = -s; sy = c; break;
+        case 2: cx = -c; sy = -s; break;
+        default: cx = s; sy = -c; break;
+    }
+    out[0] = rad * cx;
+    out[1] = rad * sy;
+    out[2] = zz;
+}

@@ -61,6 +61,9 @@ def lib() -> C.CDLL:
             getattr(L, f"oracle_forces_raw_{sfx}").argtypes = [P, u32, P, P, P, P, P, P]
             getattr(L, f"oracle_tumor_run_{sfx}").argtypes = [P, P, P, P, P, P, P, P, P, P, P, C.c_int]
             getattr(L, f"oracle_tumor_run_{sfx}").restype = C.c_int
+            getattr(L, f"oracle_pre_rule_{sfx}").argtypes = [P, u32, P, P, P, P, P, P, P, P, P, P, P, P]
+            getattr(L, f"oracle_pre_rule_{sfx}").restype = C.c_int
+            getattr(L, f"oracle_dir_from_uniforms_{sfx}").argtypes = [real, real, P]
         _lib = L
     return _lib
 
@@ -211,6 +214,33 @@ def forces_raw(desc, px, py, pz, real: str = "f32"):
     fx, fy, fz = (np.empty(n, dtype=t) for _ in range(3))
     getattr(lib(), f"oracle_forces_raw_{real}")(C.byref(desc), n, _p(px), _p(py), _p(pz), _p(fx), _p(fy), _p(fz))
     return fx, fy, fz
+
+
+def pre_rule(desc, px, py, pz, fields, real: str = "f32"):
+    """Steps 1-6 of the timestep (sort, uptake, forces + integration, one
+    sweep per field) for a rule of any form. fields: list of arrays, swept in
+    place. Returns perm, sorted keys, sorted positions before the forces and
+    integrated positions (sorted order)."""
+    t = _real(real)
+    px, py, pz = (np.ascontiguousarray(a, dtype=t) for a in (px, py, pz))
+    n = len(px)
+    perm = np.empty(n, np.uint32)
+    keys_ = np.empty(n, np.uint32)
+    sp = [np.empty(n, t) for _ in range(3)]
+    ip = [np.empty(n, t) for _ in range(3)]
+    for f in fields:
+        assert f.dtype == t and f.flags["C_CONTIGUOUS"]
+    fp = (C.c_void_p * max(1, len(fields)))(*[_p(f) for f in fields])
+    rc = getattr(lib(), f"oracle_pre_rule_{real}")(C.byref(desc), n, _p(px), _p(py), _p(pz), _p(perm), _p(keys_),
+                                                     *[_p(a) for a in sp], *[_p(a) for a in ip], fp)
+    assert rc == 0
+    return perm, keys_, sp, ip
+
+
+def dir_from_uniforms(u1, u2, real: str = "f32") -> np.ndarray:
+    out = np.empty(3, dtype=_real(real))
+    getattr(lib(), f"oracle_dir_from_uniforms_{real}")(u1, u2, _p(out))
+    return out
 
 
 class TumorState:

@@ -183,12 +183,50 @@ def test_rejects_writes_to_tissue_and_neighbours():
             _bad(f)()
 
 
-def test_chemistry_access_rejected():
+def test_chemistry_at_without_a_chemistry_grid_rejected():
     def uses_chem(self, cell: BCell):
         c = self.chemistry_at(cell.pos)
 
-    with pytest.raises(ol.CompilationError, match="chemistry"):
+    with pytest.raises(ol.CompilationError, match="setup_chemistry"):
         _bad(uses_chem)()
+
+
+def test_grid_rules_translate_and_compile():
+    """Tissue and chemistry rules (voxel loops, helper methods on voxels,
+    tuples, random draws) and chemistry_at in a cell rule: the module holds
+    the two pass kernels and their pass counts, and compiles for gfx950."""
+    from oncolytica_amd.hip._translator import ChemSchema
+    from tests.test_gpu_grid_rules import GCell, GChem, GMetrics, GTissue, Glioma
+    sim = Glioma()
+    tr = Translator(sim, CellSchema.of(GCell), TissueSchema.of(GTissue), sim._params, chem=ChemSchema.of(GChem))
+    src = tr.translate([sim.behave], [sim.count], GMetrics, tissue_rules=[sim.count_cells],
+                       chem_rules=[sim.update_pdgf])
+    for frag in ("onc_tissue_pass", "onc_chem_pass", "onc_grid_tissue_passes = 1u", "onc_grid_chem_passes = 2u",
+                 "Chem_GChem_at(ctx", "h_chem_GChem_step_pdgf", "ctx.rng.tag = 0x20000u + a.pass"):
+        assert frag in src, frag
+    _compile(src)
+
+
+def test_grid_rule_misuse_rejected():
+    from oncolytica_amd.hip._translator import ChemSchema
+    from tests.test_gpu_grid_rules import GCell, GChem, GTissue, Glioma
+
+    def writes_neighbour(self, t: GTissue):
+        for nb in t.neighbors:
+            nb.crowd = 1
+
+    def cell_writes_tissue(self, cell: GCell):
+        t: GTissue = self.tissue_at(cell.pos)
+        t.crowd = 2
+
+    sim = Glioma()
+    for fn, kind in ((writes_neighbour, "tissue"), (cell_writes_tissue, "cell")):
+        tr = Translator(sim, CellSchema.of(GCell), TissueSchema.of(GTissue), sim._params, chem=ChemSchema.of(GChem))
+        with pytest.raises(ol.CompilationError, match="read-only|may not write"):
+            if kind == "tissue":
+                tr.translate([], tissue_rules=[fn])
+            else:
+                tr.translate([fn])
 
 
 class MMetrics(ol.Metrics):
