@@ -114,7 +114,21 @@ def get_config(sim: Sim, column_names: Optional[Sequence[str]] = None, kernel: O
                                                "metric_fields": [list(m) for m in metric_fields]},
         "buffers": {"cells": "cells.bin", "fields": "fields.bin",
                     **({"chem": "chem.bin"} if sim.desc.n_chem_cols else {})},
+        "layout": layout(sim.desc, len(metric_fields)),
     }
+
+
+def layout(desc: SimDesc, n_metrics: int = 0) -> dict:
+    """The `layout` section of the reference's launch config
+    (gpu/runtime/_gpu_backend.py:196-293; framework/tests/launch_config.json
+    "layout"), for this build's storage: every record is stored as 32-bit
+    columns (structure of arrays), so a record's stride is 4 bytes per column
+    (the reference's strides are those of WGSL-aligned structs)."""
+    dims = [int(v) for v in desc.dims[:3]]
+    return {"tissue_shape": dims, "chemistry_shape": [max(1, d // 2) for d in dims],
+            "max_agents": int(desc.capacity), "cell_stride": 4 * int(desc.n_cols),
+            "tissue_stride": 4 * int(desc.n_fields), "chem_stride": 4 * int(desc.n_chem_cols),
+            "metric_stride": 4 * int(n_metrics), "storage": "structure of arrays, 32-bit columns"}
 
 
 def export_bundle(sim: Sim, export_dir: str, column_names: Optional[Sequence[str]] = None,

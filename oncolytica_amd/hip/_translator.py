@@ -1163,7 +1163,7 @@ class Translator:
                 return f"({lc} {sym} {self.conv(rc, rt, F32)})", VEC3
             if rt == VEC3 and lt in (F32, I32, U32) and sym == "*":
                 return f"({self.conv(lc, lt, F32)} * {rc})", VEC3
-            raise CompilationError("unsupported vector operation", node)
+            raise CompilationError(f"unsupported vector operation: operand types {_tname(lt)} and {_tname(rt)} do not match", node)
         isf = F32 in (lt, rt)
         if isinstance(op, ast.Div):
             return f"({self.conv(lc, lt, F32)} / {self.conv(rc, rt, F32)})", F32

@@ -100,3 +100,25 @@ def test_new_entry_points_reject_null_handles():
     assert L.onc_host_alloc(None, 64, C.byref(p)) == _abi.ERR_INVALID
     assert L.onc_host_free(None, None) == _abi.ERR_INVALID
     assert b"NULL" in L.onc_last_error()
+
+
+def test_bundle_layout_has_the_reference_launch_config_keys():
+    """The bundle's `layout` carries every key of the reference's launch
+    config layout (framework/tests/launch_config.json, written for a
+    50^3 grid and 10,000 agents), with the same shapes; strides are those of
+    this build's 32-bit columns."""
+    import json
+    from oncolytica_amd import bundle as B
+    with open(os.path.join(ROOT, "tests", "golden", "launch_layout.json")) as f:
+        ref_layout = json.load(f)
+    d = W.plain_c1().desc()
+    d.dims[:] = [50, 50, 50]
+    d.capacity = 10000
+    d.n_cols = 5            # pos.x, pos.y, pos.z, _id and the model's `id`
+    d.n_fields = 1
+    d.n_chem_cols = 1
+    lay = B.layout(d, n_metrics=1)
+    assert set(ref_layout) <= set(lay)
+    for k in ("tissue_shape", "chemistry_shape", "max_agents"):
+        assert lay[k] == ref_layout[k], k
+    assert (lay["cell_stride"], lay["tissue_stride"], lay["chem_stride"], lay["metric_stride"]) == (20, 4, 4, 4)

@@ -412,3 +412,41 @@ def test_spheroid_metrics_match_the_committed_cells():
     assert np.float32(m.age_sum).view(np.uint32) == O.metric_reduce(age, "f32")
     assert np.float32(m.oxygen_sum).view(np.uint32) == O.metric_reduce(o2, "f32")
     assert m.dividing == int(((age + np.float32(1.0) > np.float32(24.0)) & (o2 > np.float32(0.3))).sum())
+
+
+def test_backend_compile_errors_reach_load_model(monkeypatch):
+    """A hiprtc failure (ONC_ERR_COMPILE) reaches Engine.load_model as an
+    exception that carries the compiler log (framework/tests/
+    test_oncolytica.py:172-219 expect the driver's shader errors there)."""
+    from oncolytica_amd.hip import _abi
+    from oncolytica_amd.hip._translator import Translator
+
+    ctx = _abi.Context(0)
+    try:
+        with pytest.raises(_abi.OncError) as ei:
+            from oncolytica_amd.hip._sim import Module
+            Module(ctx, "extern \"C\" __global__ void k() { undeclared_thing = 1; }")
+        assert ei.value.code == _abi.ERR_COMPILE and "undeclared_thing" in str(ei.value)
+    finally:
+        ctx.close()
+
+    real = Translator.translate
+
+    def broken(self, *a, **kw):
+        return real(self, *a, **kw) + "\n__device__ int broken_decl = not_a_symbol;\n"
+
+    monkeypatch.setattr(Translator, "translate", broken)
+
+    class PCell(ol.Cell):
+        pos: ol.vec3
+        v: ol.f32 = 0.0
+
+    class PSim(ol.Simulation[MyTissue, MyChem, PCell, MyMetrics, MyParams]):
+        @ol.cell_rule
+        def rule(self, cell: PCell):
+            cell.v += 1.0
+
+    eng = ol.Engine(backend="hip")
+    eng.cells.add(PCell(pos=ol.vec3(1.0, 1.0, 1.0)))
+    with pytest.raises(RuntimeError, match="not_a_symbol"):
+        eng.load_model(PSim())

@@ -322,3 +322,13 @@ def test_metric_fields_must_be_scalars():
     tr = Translator(sim, CellSchema.of(BCell), TissueSchema.of(BTissue), sim._params)
     with pytest.raises(ol.CompilationError, match="metric field"):
         tr.translate([], [sim.c], VMetrics)
+
+
+def test_vector_scalar_mix_rejected_with_the_types():
+    """framework/tests/test_oncolytica.py:172-194: `cell.pos += 1.0` must not
+    load; the message names the mismatched types."""
+    def bad(self, cell: BCell):
+        cell.pos += 1.0
+
+    with pytest.raises(ol.CompilationError, match="types vec3 and f32"):
+        _bad(bad)()
