@@ -756,6 +756,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=80,
                     help="oracle steps in the multi-core CPU baseline sample (~10 s on 16 threads)")
     ap.add_argument("--cpu-steps-1t", type=int, default=10, help="oracle steps in the one-thread CPU sample (~10 s)")
+    ap.add_argument("--stress-only", action="store_true",
+                    help="run only the 16M-cell 512^3 stress workload and print its record as the JSON line")
     ap.add_argument("--stress", action="store_true",
                     help="also run the 16M-cell 512^3 stress workload (300 steps, mitotic burst) with its "
                          "full-size oracle parity and binary64 drift checkpoints")
@@ -808,6 +810,17 @@ def main():
 
     if args.kernels_only:
         print(json.dumps({"kernels": kernel_rooflines(ctx)}))
+        return
+
+    if args.stress_only:
+        st = stress(ctx)
+        print(json.dumps({"metric": "cell-updates/s of the stress workload", "value": st["cell_updates_per_s"],
+                          "unit": "cell-updates/s", "n_gpus": 1, "steps": st["steps"],
+                          "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": st["workload"], "cells_initial": st["cells_initial"],
+                                     "grid": st["grid"], "fields": st["fields"], "burst_steps": st["burst_steps"],
+                                     "device": info.name.decode(), "cus": info.compute_units},
+                          "stress": st}))
         return
 
     w = W.large_c2(n_cells=args.cells)

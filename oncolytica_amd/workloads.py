@@ -141,7 +141,13 @@ def plain_c1(seed: int = 42) -> Workload:
 
 
 def large_c2(seed: int = 42, n_cells: int = 1_000_000) -> Workload:
-    """1M cells in a radius-80 sphere (+N(0, 0.3^2) jitter), 256^3, O2 + drug."""
+    """1M cells in a radius-80 sphere (+N(0, 0.3^2) jitter), 256^3, O2 + drug.
+
+    The configuration does not give a cell radius; 0.4 is used (contact
+    distance 0.8) so that the adhesion taper between contact and the 1.0
+    interaction radius is not empty: with the 4,096-cell case's 0.5 the
+    contact distance equals the interaction radius and k_adh would act on no
+    pair."""
     return Workload(
         name="large_1m_256",
         dims=(256, 256, 256), dx=1.0, key_format=_abi.KEY_MORTON32, capacity=2 * n_cells, seed=seed,
