@@ -195,6 +195,34 @@ __device__ __forceinline__ onc_u32 onc_morton(onc_u32 x, onc_u32 y, onc_u32 z, i
     return k;
 }
 
+// Morton keys of a voxel's 27 neighbours from its own key by dilated
+// arithmetic: the x (y, z) bits of a key, kept under their mask M, form a
+// dilated integer with x - 1 = (kx - 1) & M and x + 1 = ((kx | ~M) + 1) & M
+// (the borrow or carry runs through the other axes' bits). The key of
+// (x + dx, y + dy, z + dz) is x[dx + 1] | y[dy + 1] | z[dz + 1]: a few
+// operations per neighbour instead of three bit interleavings each. Keys of
+// coordinates outside [0, dims) are meaningless and must not be used.
+struct onc_nkeys {
+    onc_u32 x[3], y[3], z[3];
+};
+__device__ __forceinline__ onc_nkeys onc_neighbor_keys(onc_u32 k, int fmt) {
+    const onc_u32 XM = fmt == 32 ? 0x49249249u : 0x09249249u;
+    const onc_u32 YM = fmt == 32 ? 0x92492492u : 0x12492492u;
+    const onc_u32 ZM = 0x24924924u;
+    onc_nkeys n;
+    const onc_u32 kx = k & XM, ky = k & YM, kz = k & ZM;
+    n.x[0] = (kx - 1u) & XM;
+    n.x[1] = kx;
+    n.x[2] = ((kx | ~XM) + 1u) & XM;
+    n.y[0] = (ky - 1u) & YM;
+    n.y[1] = ky;
+    n.y[2] = ((ky | ~YM) + 1u) & YM;
+    n.z[0] = (kz - 1u) & ZM;
+    n.z[1] = kz;
+    n.z[2] = ((kz | ~ZM) + 1u) & ZM;
+    return n;
+}
+
 __device__ __forceinline__ onc_u32 onc_cell_key(const onc_dsim& S, float px, float py, float pz) {
     return onc_morton((onc_u32)onc_vox1(px, S.dx, S.dims[0]), (onc_u32)onc_vox1(py, S.dx, S.dims[1]),
                       (onc_u32)onc_vox1(pz, S.dx, S.dims[2]), S.key_format);
@@ -445,16 +473,20 @@ __device__ __forceinline__ void onc_for_neighbors(const onc_dsim& S, onc_u32 epo
     const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
               vz = onc_vox1(pz, S.dx, S.dims[2]);
     const uint4* bins = (const uint4*)S.bins;
+    const onc_nkeys nk = onc_neighbor_keys(onc_morton((onc_u32)vx, (onc_u32)vy, (onc_u32)vz, S.key_format),
+                                           S.key_format);
     for (int oz = -1; oz <= 1; oz++) {
         const int nz = vz + oz;
         if (nz < 0 || nz >= S.dims[2]) continue;
+        // a select, not nk.z[oz + 1]: a runtime index would put nk in scratch memory
+        const onc_u32 zk = oz < 0 ? nk.z[0] : (oz == 0 ? nk.z[1] : nk.z[2]);
         uint4 b[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             const int ny = vy + q / 3 - 1, nx = vx + q % 3 - 1;
             const bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
             b[q] = make_uint4(0u, 0u, 0u, 0u);
-            if (in) b[q] = bins[onc_morton((onc_u32)nx, (onc_u32)ny, (onc_u32)nz, S.key_format)];
+            if (in) b[q] = bins[nk.x[q % 3] | nk.y[q / 3] | zk];
         }
 #pragma unroll
         for (int q = 0; q < 9; q++) {
@@ -477,9 +509,12 @@ __device__ __forceinline__ void onc_for_neighbor_pos4(const onc_dsim& S, onc_u32
     const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
               vz = onc_vox1(pz, S.dx, S.dims[2]);
     const uint4* __restrict__ bins = (const uint4*)S.bins;
+    const onc_nkeys nk = onc_neighbor_keys(onc_morton((onc_u32)vx, (onc_u32)vy, (onc_u32)vz, S.key_format),
+                                           S.key_format);
     for (int oz = -1; oz <= 1; oz++) {
         const int nz = vz + oz;
         if (nz < 0 || nz >= S.dims[2]) continue;
+        const onc_u32 zk = oz < 0 ? nk.z[0] : (oz == 0 ? nk.z[1] : nk.z[2]);
         onc_u32 st[9], pre[10];
         pre[0] = 0u;
 #pragma unroll
@@ -487,7 +522,7 @@ __device__ __forceinline__ void onc_for_neighbor_pos4(const onc_dsim& S, onc_u32
             const int ny = vy + q / 3 - 1, nx = vx + q % 3 - 1;
             const bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
             uint4 b = make_uint4(0u, 0u, 0u, 0u);
-            if (in) b = bins[onc_morton((onc_u32)nx, (onc_u32)ny, (onc_u32)nz, S.key_format)];
+            if (in) b = bins[nk.x[q % 3] | nk.y[q / 3] | zk];
             const onc_u32 c = b.z == epoch ? b.y - b.x : 0u;
             st[q] = b.x;
             pre[q + 1] = pre[q] + c;

@@ -779,16 +779,20 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, fl
     const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
               vz = onc_vox1(pz, S.dx, S.dims[2]);
     const uint4* __restrict__ bins = (const uint4*)S.bins;
+    const onc_nkeys nk = onc_neighbor_keys(onc_morton((onc_u32)vx, (onc_u32)vy, (onc_u32)vz, S.key_format),
+                                           S.key_format);
     for (int oz = -1; oz <= 1; oz++) {
         const int nz = vz + oz;
         if (nz < 0 || nz >= S.dims[2]) continue;
+        // a select, not nk.z[oz + 1]: a runtime index would put nk in scratch memory
+        const onc_u32 zk = oz < 0 ? nk.z[0] : (oz == 0 ? nk.z[1] : nk.z[2]);
         uint4 b[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             const int ny = vy + q / 3 - 1, nx = vx + q % 3 - 1;
             const bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
             b[q] = make_uint4(0u, 0u, 0u, 0u);
-            if (in) b[q] = bins[onc_morton((onc_u32)nx, (onc_u32)ny, (onc_u32)nz, S.key_format)];
+            if (in) b[q] = bins[nk.x[q % 3] | nk.y[q / 3] | zk];
         }
         onc_u32 nr = 0;
 #pragma unroll

@@ -888,15 +888,18 @@ class Translator:
                   f"make_float4({sv}.pos.x, {sv}.pos.y, {sv}.pos.z, 0.0f);")
         self.emit("const int _vx = onc_vox1(_c.x, _S.dx, _S.dims[0]), _vy = onc_vox1(_c.y, _S.dx, _S.dims[1]), "
                   "_vz = onc_vox1(_c.z, _S.dx, _S.dims[2]);")
+        self.emit("const onc_nkeys _nk = onc_neighbor_keys(onc_morton((onc_u32)_vx, (onc_u32)_vy, (onc_u32)_vz, "
+                  "_S.key_format), _S.key_format);")
         self.emit("for (int _oz = -1; _oz <= 1; _oz++) {")
         self.indent += 1
         self.emit("const int _nz = _vz + _oz;")
         self.emit("if (_nz < 0 || _nz >= _S.dims[2]) continue;")
+        self.emit("const onc_u32 _zk = _oz < 0 ? _nk.z[0] : (_oz == 0 ? _nk.z[1] : _nk.z[2]);")
         self.emit(f"for (int {q} = 0; {q} < 9; {q}++) {{")
         self.indent += 1
         self.emit(f"const int _ny = _vy + {q} / 3 - 1, _nx = _vx + {q} % 3 - 1;")
         self.emit("if (_nx < 0 || _ny < 0 || _nx >= _S.dims[0] || _ny >= _S.dims[1]) continue;")
-        self.emit(f"const uint4 {b} = ((const uint4*)_S.bins)[onc_morton((onc_u32)_nx, (onc_u32)_ny, (onc_u32)_nz, _S.key_format)];")
+        self.emit(f"const uint4 {b} = ((const uint4*)_S.bins)[_nk.x[{q} % 3] | _nk.y[{q} / 3] | _zk];")
         self.emit(f"if ({b}.z != ctx.epoch) continue;")
         self.emit(f"for (onc_u32 {j} = {b}.x; {j} < {b}.y; {j}++) {{")
         self.indent += 1
