@@ -1029,7 +1029,7 @@ __device__ __forceinline__ float onc_sweep1(const onc_dfield& fd, float c, float
 
 // Register-blocked sweep for rows whose length is a multiple of 4: a thread
 // owns 4 consecutive voxels of a row (float4 loads / stores) and marches sbz
-// planes along z; rows y-1, y+1 and plane z+1 come in as float4. A block of
+// planes along z; rows y-1, y+1 and planes z+1, z+2 come in as float4. A block of
 // NT threads tiles tw = onc_stencil_tw(X/4, min(sbx, NT)) float4 columns by
 // NT / tw rows,
 // so each row segment of W = min(tw, 64) lanes holds consecutive float4 of
@@ -1067,9 +1067,13 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
     const bool hxm = x0 > 0, hxp = x0 + 4 < X, hym = y > 0, hyp = y < Y - 1;
     const bool ldl = hxm && sl == 0, ldr = hxp && sl == W - 1;
     const bool dir = fd.bc == 0, psrc = fd.bc == 1;
+    // The centre column runs two planes ahead of the plane being computed (its
+    // z+1 neighbour must already be in registers), the y rows and x halos one
+    // plane ahead: every load has a whole iteration to arrive.
     size_t v = row + XY * (size_t)z0;
     float4 c = *(const float4*)(in + v);
     float4 cm = z0 > 0 ? *(const float4*)(in + v - XY) : c;
+    float4 cp = z0 + 1 < Z ? *(const float4*)(in + v + XY) : c;
     float4 ym = hym ? *(const float4*)(in + v - (size_t)X) : c;
     float4 yp = hyp ? *(const float4*)(in + v + (size_t)X) : c;
     float lx = ldl ? in[v - 1] : 0.0f;
@@ -1078,10 +1082,11 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
         const bool nxt = z + 1 < Z;
         const bool more = z + 1 < z1;
         const size_t vn = v + XY;
-        const float4 cp = nxt ? *(const float4*)(in + vn) : c;
+        float4 cpp = cp;
         float4 nym = cp, nyp = cp;
         float nlx = 0.0f, nrx = 0.0f;
         if (more) {
+            if (z + 2 < Z) cpp = *(const float4*)(in + vn + XY);
             if (hym) nym = *(const float4*)(in + vn - (size_t)X);
             if (hyp) nyp = *(const float4*)(in + vn + (size_t)X);
             if (ldl) nlx = in[vn - 1];
@@ -1116,6 +1121,7 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
         onc_st_nt((float4*)(out + v), o);
         cm = c;
         c = cp;
+        cp = cpp;
         ym = nym;
         yp = nyp;
         lx = nlx;
@@ -1163,17 +1169,19 @@ __device__ __forceinline__ void onc_stencil_block(const onc_kargs& a, onc_u32 b)
     size_t v = col + XY * (size_t)z0;
     float cc = in[v];
     float cm = z0 > 0 ? in[v - XY] : cc;
+    float cp = z0 + 1 < Z ? in[v + XY] : cc;
     float xm = hxm ? in[v - 1] : cc;
     float xp = hxp ? in[v + 1] : cc;
     float ym = hym ? in[v - (size_t)X] : cc;
     float yp = hyp ? in[v + (size_t)X] : cc;
     for (int z = z0; z < z1; z++, v += XY) {
-        // issue plane z+1 loads before computing plane z
+        // issue the loads of plane z+1 (its centre: plane z+2) before computing plane z
         const bool nxt = z + 1 < Z;
         const size_t vn = v + XY;
-        const float cp = nxt ? in[vn] : cc;
+        float cpp = cp;
         float nxm = 0.0f, nxp = 0.0f, nym = 0.0f, nyp = 0.0f;
         if (nxt && z + 1 < z1) {
+            if (z + 2 < Z) cpp = in[vn + XY];
             nxm = hxm ? in[vn - 1] : 0.0f;
             nxp = hxp ? in[vn + 1] : 0.0f;
             nym = hym ? in[vn - (size_t)X] : 0.0f;
@@ -1200,6 +1208,7 @@ __device__ __forceinline__ void onc_stencil_block(const onc_kargs& a, onc_u32 b)
         xp = hxp ? nxp : cp;
         ym = hym ? nym : cp;
         yp = hyp ? nyp : cp;
+        cp = cpp;
     }
 }
 

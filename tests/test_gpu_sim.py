@@ -110,6 +110,9 @@ def test_deterministic_repeat_and_graph(ctx):
 
 
 def test_batch_equals_alone(ctx):
+    """Five ensemble members stepped as one batch (graph replays of 5 and 7
+    steps) equal each member stepped alone, and each equals the oracle run
+    for the same 12 steps."""
     ws = W.ensemble_c3(n_sims=5, scale=0.05)
     alone = []
     for w in ws:
@@ -129,11 +132,15 @@ def test_batch_equals_alone(ctx):
     b.step(5, graph=True)
     b.step(7, graph=True)
     b.sync()
-    for s, (cols, f) in zip(sims, alone):
+    for k, (w, s, (cols, f)) in enumerate(zip(ws, sims, alone)):
         got = s.download_cells()
         for a, g in zip(cols, got):
             np.testing.assert_array_equal(a.view(np.uint32), g.view(np.uint32))
         np.testing.assert_array_equal(f.view(np.uint32), s.download_field(0).view(np.uint32))
+        st = O.TumorState(w.desc(), W.initial_cells(w))
+        st.run(12)
+        print(f"batch member {k}: {len(got[0])} cells vs oracle", flush=True)
+        compare(12, got, [s.download_field(q) for q in range(w.desc().n_fields)], st)
     b.close()
     for s in sims:
         s.close()
