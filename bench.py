@@ -16,7 +16,11 @@ Prints ONE JSON line with:
   roofline   the dominant kernel's algorithmic bytes per launch / its mean
              event-timed duration in the timed region, against 8 TB/s;
   cpu_baseline  the CPU oracle (binary32 C restatement, one thread) on a
-             bounded sample of the same workload.
+             bounded sample of the same workload;
+  long_run   the same workload stepped --long-steps (1000) times from its
+             initial state as graph replays, timed whole: the population
+             grows, then collapses to ~0.1M cells once the spheroid's core
+             turns hypoxic.
 """
 from __future__ import annotations
 
@@ -761,6 +765,9 @@ def main():
     ap.add_argument("--stress", action="store_true",
                     help="also run the 16M-cell 512^3 stress workload (300 steps, mitotic burst) with its "
                          "full-size oracle parity and binary64 drift checkpoints")
+    ap.add_argument("--long-steps", type=int, default=1000,
+                    help="steps of the long run of the same workload (graph replays from its initial state, "
+                         "through the hypoxic collapse to ~0.1M cells; 0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-sims", type=int, default=64, help="sims of the batched uneven ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
@@ -977,6 +984,31 @@ def main():
                            "bound": "hbm" if ph == "stencil" else "gather latency"}
     roof["parts"] = parts
 
+    # the long run: the workload stepped --long-steps times from its initial
+    # state as graph replays, the whole run timed (the population grows, then
+    # collapses once the spheroid's core turns hypoxic; most of the run is
+    # spent at ~0.1M cells, where every launch sits on its latency floor)
+    long_run = None
+    if args.long_steps > 0:
+        lsim, lbatch = fresh()
+        lbatch.step(1, graph=True)
+        lbatch.sync()
+        lcu0 = cell_updates(lsim)
+        l0 = time.perf_counter()
+        lbatch.step(args.long_steps - 1, graph=True)
+        lbatch.sync()
+        l1 = time.perf_counter()
+        lcu = cell_updates(lsim) - lcu0
+        ln_final, _, _ = lsim.counts()
+        long_run = {"steps": args.long_steps, "timed_steps": args.long_steps - 1,
+                    "value": lcu / (l1 - l0), "unit": "cell-updates/s",
+                    "ms_per_step": 1000.0 * (l1 - l0) / max(1, args.long_steps - 1),
+                    "mean_live_cells": lcu / max(1, args.long_steps - 1), "final_live_cells": ln_final}
+        _log(f"long run: {args.long_steps} steps, {long_run['value'] / 1e6:.0f} M cell-updates/s, "
+             f"{long_run['ms_per_step'] * 1000:.1f} us/step, mean {long_run['mean_live_cells']:.0f} cells")
+        lbatch.close()
+        lsim.close()
+
     met = metrics_leg(ctx, desc, cols0, Sim)
     bun = bundle_leg(ctx, desc, cols0, Sim)
     kern = None if args.no_kernels else kernel_rooflines(ctx)
@@ -1015,6 +1047,7 @@ def main():
         "ensemble": ens,
         "serving": served,
         "stress": stress_out,
+        "long_run": long_run,
     }
     out["graph_value"] = (gcu1 - gcu0) / (g1 - g0) if gsteps else None
     print(json.dumps(out), flush=True)
