@@ -415,36 +415,51 @@ __device__ __forceinline__ void onc_for_cells_wave(const onc_u32* cell_prefix, o
 }
 
 // Walk for a grid of nblk one-wave blocks (a multiple of 8) that may hold
-// fewer waves than chunks.
-// Chunks go to XCDs in groups of 8 (512 consecutive cells, one spatial
-// neighbourhood): group g belongs to XCD g % 8, so every XCD walks the whole
-// cell range at the same pace. Wave i of XCD x (block id 8 i + x, the
-// dispatcher's round-robin) takes XCD x's chunks i, i + W_x, i + 2 W_x, ...,
-// where the m-th chunk of XCD x is 8 (x + 8 floor(m / 8)) + m % 8: with at
-// least one wave per chunk each wave takes one chunk; with fewer, a wave's
-// chunks are strided through its XCD's share, which spreads dense and sparse
-// regions evenly over the waves.
+// fewer waves than chunks. XCD x (blocks 8 i + x, the dispatcher's
+// round-robin) takes the x-th eighth of every sim's 64-cell chunks: each
+// eighth is one contiguous stretch of the sim's Morton order, so the bin
+// entries and positions an XCD reads (its cells and their neighbours) stay in
+// its own L2 instead of being fetched by all eight, and every XCD holds the
+// same share of every sim, dense or sparse. Wave i of XCD x takes its XCD's
+// chunks i, i + W_x, i + 2 W_x, ... in the order (sim, chunk): with at least
+// one wave per chunk each wave takes one chunk. `f(s, i)` gets the sim and
+// the cell's index in it; lanes past a sim's last cell do not call it.
 template <class F>
 __device__ __forceinline__ void onc_for_chunks_xcd(const onc_u32* cell_prefix, onc_u32 nsims, onc_u32 blk,
                                                    onc_u32 nblk, F f) {
-    const onc_u32 total = cell_prefix[nsims];
     const onc_u32 lane = threadIdx.x & 63u;
-    const onc_u32 nchunks = (total + 63u) >> 6;
     const onc_u32 x = blk & 7u, i = blk >> 3;
     const onc_u32 wx = nblk >> 3;  // nblk is a multiple of 8 (>= 8): every XCD class has waves
-    for (onc_u32 m = i;; m += wx) {
-        const onc_u32 c = 8u * (x + 8u * (m >> 3)) + (m & 7u);
-        if (c >= nchunks) break;
-        const onc_u32 base = c * 64u;
-        const onc_u32 g = base + lane;
-        const int s0 = onc_find_seg_wave(cell_prefix, nsims, base);
-        const onc_u32 lo = cell_prefix[s0], hi = cell_prefix[s0 + 1];
-        if (base + 63u < hi) {
-            f(s0, g - lo);
-        } else if (g < total) {
-            const int s = onc_find_seg(cell_prefix, nsims, g);
-            f(s, g - cell_prefix[s]);
+    onc_u32 m = i;       // this wave's next chunk among XCD x's chunks
+    onc_u32 before = 0;  // XCD x's chunks in the sims of earlier groups
+    for (onc_u32 g0 = 0; g0 < nsims; g0 += 64u) {
+        // lane l: sim g0 + l, its cell count, and its chunks [lo, lo + cnt) for XCD x
+        const onc_u32 s = g0 + lane;
+        onc_u32 n = 0, lo = 0, cnt = 0;
+        if (s < nsims) {
+            n = cell_prefix[s + 1] - cell_prefix[s];
+            const onc_u32 ch = (n + 63u) >> 6;
+            lo = (onc_u32)(((unsigned long long)ch * x) >> 3);
+            cnt = (onc_u32)(((unsigned long long)ch * (x + 1u)) >> 3) - lo;
         }
+        onc_u32 incl = cnt;  // inclusive prefix over the lanes
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const onc_u32 t = __shfl_up(incl, d);
+            if ((int)lane >= d) incl += t;
+        }
+        const onc_u32 gtot = onc_uniform(__shfl(incl, 63));
+        for (; m < before + gtot; m += wx) {
+            const onc_u32 r = m - before;
+            // the owning lane: the first whose inclusive prefix exceeds r
+            const int sl = __popcll(__ballot(incl <= r));
+            const onc_u32 sim = onc_uniform(g0 + (onc_u32)sl);
+            const onc_u32 lc = onc_uniform(__shfl(lo, sl) + (r - (__shfl(incl, sl) - __shfl(cnt, sl))));
+            const onc_u32 ns = onc_uniform(__shfl(n, sl));
+            const onc_u32 cell = lc * 64u + lane;
+            if (cell < ns) f((int)sim, cell);
+        }
+        before += gtot;
     }
 }
 
