@@ -149,6 +149,7 @@ struct onc_kargs {
     int sbx, sby, sbz;    // stencil block shape
     onc_u32 fs_sgroups;   // fused forces + sweep launch: groups of 8 blocks that sweep
     onc_u32 ONC_GLOBAL* err;         // sticky error word (bounded waits that expired)
+    onc_u32 xcd_tiles;    // sort passes: tiles whose blocks are all resident at once (0: none)
 };
 
 // ---- small helpers ---------------------------------------------------------

@@ -59,6 +59,16 @@ hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s);
 hipError_t onc_launch_morton(const onc_kargs& a, int grid, hipStream_t s);
 hipError_t onc_launch_hist_only(const onc_kargs& a, int items, int grid, hipStream_t s);
 hipError_t onc_launch_sort_pass(const onc_kargs& a, int items, int pass, int grid, hipStream_t s);
+// Blocks of the radix pass with `items` cells per thread that one CU holds at
+// once (0 if the runtime cannot tell).
+int onc_sort_pass_blocks_per_cu(int items);
+// Tiles up to which a radix pass takes XCD-contiguous tile ranges: three
+// quarters of the blocks the chip holds at once, so every tile's block is
+// resident together even with some of the chip taken by other work (the
+// library otherwise assumes the device to itself while a step runs).
+inline unsigned onc_sort_xcd_tiles(int items, int cus) {
+    return (unsigned)(onc_sort_pass_blocks_per_cu(items) * cus * 3 / 4);
+}
 // onc_forces: pack positions and keys, bin entries (epoch 1), then the raw
 // force walk of one sim table entry S (device) with a.cell_prefix = {0, n};
 // t0 / t1 (may be null) bracket the force kernel alone

@@ -413,14 +413,22 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
     const onc_u32 shift = 8u * (onc_u32)pass;
     const onc_u64 lt_mask = (1ull << lane) - 1ull;
     onc_u32* status = a.sort_status + (size_t)pass * a.max_tiles * 256u;
-    // Static tile order: block b takes tile b (the launch has one block per
-    // tile). A tile only waits (look-back) on lower tiles, whose blocks were
-    // dispatched before it, so the lowest unfinished tile can always run.
-    // (A shared tile counter would also give that order, but same-address
-    // device atomics serialise at one memory channel and stall every other
-    // access queued behind them.) With the grid covering every tile, the
-    // loop below runs at most once per block.
-    for (onc_u32 g = blockIdx.x; g < total; g += gridDim.x) {
+    // Tile order. The launch has one block per tile, so the loop below runs
+    // at most once per block. When the step's tiles are no more than the
+    // blocks the chip holds at once (a.xcd_tiles), every tile's block is
+    // resident together and any order lets every look-back wait end: then
+    // block b takes tile onc_xcd_block(b, total), so each XCD ranks and
+    // scatters one contiguous range of tiles, and the neighbouring runs of a
+    // digit that consecutive tiles write share lines in one L2 (tiles
+    // round-robin over the XCDs write every line back as eight partial
+    // copies). Otherwise block b takes tile b: a tile only waits on lower
+    // tiles, whose blocks were dispatched before it, so the lowest unfinished
+    // tile can always run. (A shared tile counter would also give that order,
+    // but same-address device atomics serialise at one memory channel and
+    // stall every other access queued behind them.)
+    const bool xmap = total <= a.xcd_tiles;
+    for (onc_u32 bb = blockIdx.x; bb < total; bb += gridDim.x) {
+        const onc_u32 g = xmap ? onc_uniform(onc_xcd_block(bb, total)) : bb;
         for (int i = tid; i < 4 * 256; i += 256) s_wc[i] = 0u;
         __syncthreads();
         const int s = onc_find_seg_wave(a.tile_prefix, a.nsims, g);
@@ -1734,6 +1742,19 @@ hipError_t onc_launch_hist_only(const onc_kargs& a, int items, int grid, hipStre
 hipError_t onc_launch_sort_pass(const onc_kargs& a, int items, int pass, int grid, hipStream_t s) {
     ONC_DISPATCH_ITEMS(items, k_sort_pass, grid, s, a, pass);
     return hipGetLastError();
+}
+int onc_sort_pass_blocks_per_cu(int items) {
+    int nb = 0;
+    hipError_t e;
+    if (items == 4)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sort_pass<4>, 256, 0);
+    else if (items == 8)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sort_pass<8>, 256, 0);
+    else if (items == 12)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sort_pass<12>, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sort_pass<16>, 256, 0);
+    return e == hipSuccess ? nb : 0;
 }
 hipError_t onc_launch_permute_bins(const onc_kargs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_permute_bins, dim3(grid), dim3(256), 0, s, a);

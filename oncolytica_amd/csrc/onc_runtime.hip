@@ -954,6 +954,7 @@ static int batch_rebuild(onc_batch* b) {
     a.passes = (onc_u32)passes;
     a.tile = (onc_u32)tile;
     a.max_tiles = (onc_u32)max_tiles;
+    a.xcd_tiles = onc_sort_xcd_tiles(b->items, b->ctx->cus);
     a.ctile = (onc_u32)ctile;
     a.max_ctiles = (onc_u32)max_ctiles;
     a.tile_prefix = b->d_small;
@@ -1581,6 +1582,7 @@ extern "C" int onc_sort_pairs(onc_ctx* c, const uint32_t* kin, const uint32_t* v
     a.passes = (onc_u32)passes;
     a.tile = (onc_u32)tile;
     a.max_tiles = (onc_u32)tiles;
+    a.xcd_tiles = onc_sort_xcd_tiles(items, c->cus);
     onc_u32* small = (onc_u32*)(B + o_small);
     a.ctile = (onc_u32)tile;
     a.max_ctiles = (onc_u32)tiles;
