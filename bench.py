@@ -534,15 +534,23 @@ def stress(ctx, steps: int = 300, burst=(100, 110), checkpoints=(0, 105)):
     return out
 
 
+# the driver's command under a kernel trace (`--summarize-timed DIR` reduces
+# it, and `--summarize DIR` takes its kernel table from it). The serving leg
+# runs after the timed region and is left out: under the kernel trace the
+# process took a host-side SIGSEGV inside the HIP runtime during that leg, and
+# during the 1000-step eager pass of the default command, which both run to
+# completion without the profiler.
+DRIVER_ARGS = "--gpus 1 --steps 20 --warmup 5"
+TIMED_RUN = ("rocprofv3 --kernel-trace --stats -d {d}/kt -o run -- python3 bench.py " + DRIVER_ARGS +
+             " --serve-seconds 0 > {d}/bench.json")
 # rocprofv3 runs whose outputs `--summarize DIR` reduces (each under its own
-# subdirectory of DIR, written with `-d DIR/<sub> -o run`)
+# subdirectory of DIR, written with `-d DIR/<sub> -o run`): the traced driver
+# command and two byte-counter runs of the same steps
+_PMC_ARGS = DRIVER_ARGS + " --no-cpu-baseline --ensemble-sims 0 --serve-seconds 0 --long-steps 0 --no-kernels --calibrate"
 PROFILE_RUNS = {
-    "kt": "rocprofv3 --kernel-trace --stats -d {d}/kt -o run -- python3 bench.py "
-          "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0",
-    "FETCH_SIZE": "rocprofv3 --pmc FETCH_SIZE -d {d}/FETCH_SIZE -o run -- python3 bench.py --steps 100 "
-                  "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0 --no-kernels --calibrate",
-    "WRITE_SIZE": "rocprofv3 --pmc WRITE_SIZE -d {d}/WRITE_SIZE -o run -- python3 bench.py --steps 100 "
-                  "--no-cpu-baseline --ensemble-sims 0 --serve-seconds 0 --no-kernels --calibrate",
+    "kt": TIMED_RUN,
+    "FETCH_SIZE": "rocprofv3 --pmc FETCH_SIZE -d {d}/FETCH_SIZE -o run -- python3 bench.py " + _PMC_ARGS,
+    "WRITE_SIZE": "rocprofv3 --pmc WRITE_SIZE -d {d}/WRITE_SIZE -o run -- python3 bench.py " + _PMC_ARGS,
 }
 KERNEL_PHASE = {"k_plan": "plan", "k_morton": "morton", "k_sort_pass": "sort", "k_permute_bins": "permute_bins",
                 "k_forces": "forces", "k_stencil": "stencil", "k_rule_tumor": "rule", "k_compact": "compact",
@@ -632,15 +640,6 @@ def summarize(d: str, workload: str = "large_1m_256", outdir: str = None):
     with open(md, "w") as f:
         f.write("\n".join(lines) + "\n")
     print(f"wrote {tj} and {md}")
-
-
-# the driver's command under a kernel trace (`--summarize-timed DIR` reduces
-# it). The serving leg runs after the timed region and is left out: under the
-# kernel trace the process took a host-side SIGSEGV inside the HIP runtime
-# during that leg, which runs to completion without the profiler.
-DRIVER_ARGS = "--gpus 1 --steps 20 --warmup 5"
-TIMED_RUN = ("rocprofv3 --kernel-trace --stats -d {d}/kt -o run -- python3 bench.py " + DRIVER_ARGS +
-             " --serve-seconds 0 > {d}/bench.json")
 
 
 def summarize_timed(d: str, outdir: str = None, workload: str = "large_1m_256"):
