@@ -88,7 +88,8 @@ struct onc_dsim {
     onc_u32 n_live, par, step, next_id;
     onc_u32 frozen;  // step >= stop: this step leaves the sim untouched
     onc_u32 split;   // cells [0, split) are in st[par], [split, n_live) in dau2
-    onc_u32 pad_[2];
+    onc_u32 job0;    // the sweep job of this sim's field 0 in its batch (field f: job0 + f)
+    onc_u32 pad_;
     float cell_radius, R, k_rep, k_adh, mobility, max_disp;
     int n_fields, n_cols;
     onc_dfield fields[ONC_DEV_MAX_FIELDS];
@@ -115,7 +116,15 @@ struct onc_dsim {
     onc_u32 chem_passes;
     onc_u32 pad_c;
     onc_u32 ONC_GLOBAL* chem[2][ONC_DEV_MAX_CHEM];
+    // Uptake stamps of cubes of 2^ONC_QCUBE voxels per side, indexed by
+    // Morton key >> (3 ONC_QCUBE): the step + 1 in which the permute kernel
+    // last changed a field voxel of the cube (0: never). The sweep's quiet
+    // tiles read them (onc_stencil_vec4).
+    onc_u32 ONC_GLOBAL* qstamp;
 };
+#ifndef ONC_QCUBE
+#define ONC_QCUBE 3
+#endif
 
 // Arguments of the grid-rule kernels of a translated module (onc_tissue_pass,
 // onc_chem_pass): one launch per pass over every sim of the batch (blockIdx.y
@@ -150,6 +159,10 @@ struct onc_kargs {
     onc_u32 fs_sgroups;   // fused forces + sweep launch: groups of 8 blocks that sweep
     onc_u32 ONC_GLOBAL* err;         // sticky error word (bounded waits that expired)
     onc_u32 xcd_tiles;    // sort passes: tiles whose blocks are all resident at once (0: none)
+    // quiet sweep tiles: {state, value} per sweep block and field buffer,
+    // [2][nqb][2] (null: every tile is swept)
+    onc_u32 ONC_GLOBAL* quiet;
+    onc_u32 nqb;
 };
 
 // ---- small helpers ---------------------------------------------------------

@@ -686,6 +686,12 @@ __device__ __forceinline__ bool onc_is_fixed(const onc_dfield& f, const int* d, 
     return fx * fx + fy * fy <= r * r;
 }
 
+// Quiet-tile states (onc_kargs::quiet)
+#define ONC_Q_NONE 0u      // not known to be uniform
+#define ONC_Q_COMPUTED 1u  // computed, and every output came out equal
+#define ONC_Q_FILLED 2u    // written as a uniform tile without reading the input
+#define ONC_Q_KEPT 3u      // already held the value: neither read nor written
+
 // ---------------------------------------------------------------------------
 // Gather every state column into sorted order (current -> other buffer),
 // write bin start/end of every occupied key, and apply the per-voxel uptake
@@ -725,6 +731,20 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 #pragma unroll
             for (int q = 0; q < 8; q++)
                 if (c0 + q < S.n_cols) S.st[par ^ 1u][c0 + q][i] = v[q];
+        }
+        // Stamp the cube of each occupied voxel when the sim takes uptake (the
+        // sweep's quiet tiles read the stamps; a voxel whose uptake the fixed
+        // boundary skips is stamped too, which only keeps its tile swept):
+        // the lanes of one call hold consecutive cells (lanes 0..last active),
+        // so a cube's cells form a stretch of lanes and its first lane stores.
+        if (a.quiet) {
+            bool up = false;
+            for (int f = 0; f < S.n_fields; f++) up = up || S.fields[f].uptake != 0.0f;
+            if (up) {
+                const onc_u32 cube = k >> (3 * ONC_QCUBE);
+                const onc_u32 cprev = __shfl_up(cube, 1);
+                if ((threadIdx.x & 63u) == 0u || cprev != cube) S.qstamp[cube] = S.step + 1u;
+            }
         }
         // run lengths of equal keys from the wave's run starts: the lanes of
         // one call hold consecutive cells (lanes 0..last active), and a sim
@@ -1035,6 +1055,16 @@ __device__ __forceinline__ float onc_sweep1(const onc_dfield& fd, float c, float
     return onc_clampf(w, fd.lo, fd.hi);
 }
 
+// Waves of an NT-thread float4 sweep block with at least one voxel: xrem
+// float4 columns and yrem rows of the tile lie in the grid; a wave holds a
+// voxel iff its first lane does (lanes run along a row, rows of tw lanes).
+__device__ __forceinline__ onc_u32 onc_stencil_live_waves(int tw, int th, int xrem, int yrem, int NT) {
+    onc_u32 n = 0;
+    for (int k = 0; k < NT / 64; k++)
+        if ((64 * k) / tw < min(yrem, th) && (64 * k) % tw < min(xrem, tw)) n++;
+    return n;
+}
+
 // Register-blocked sweep for rows whose length is a multiple of 4: a thread
 // owns 4 consecutive voxels of a row (float4 loads / stores) and marches sbz
 // planes along z; rows y-1, y+1 and planes z+1, z+2 come in as float4. A block of
@@ -1044,9 +1074,23 @@ __device__ __forceinline__ float onc_sweep1(const onc_dfield& fd, float c, float
 // one row: the x halo values pass between neighbouring lanes by shuffle, and
 // only the segment's end lanes load them (one scalar load per segment
 // instead of two per lane: the scalar halo loads, not HBM, limited the sweep).
+//
+// Quiet tiles (a.quiet set): the sweep keeps a state word and a value word
+// per tile (the voxels of one block) and field buffer. State ONC_Q_NONE: not
+// known to be uniform; any other state: every voxel of the tile holds the
+// value in that buffer. A block whose tile and face-neighbour tiles all hold
+// the same value v in the input buffer computes every output as v when v is a
+// fixed point of the update (the update of a voxel whose seven stencil values
+// are v gives v, bit for bit), a Dirichlet face holds v, and no source voxel
+// lies in the tile: then it writes v (or nothing, when the output buffer's
+// tile already holds v) without reading the input. Every other block computes
+// its tile and records whether the outputs came out uniform. The permute
+// kernel's uptake stamps the cubes it changes with the step (qstamp);
+// host writes, step counter changes and batch rebuilds reset every tile to
+// ONC_Q_NONE. The results are the bits of the full sweep.
 template <int NT>
-__device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_dfield& fd, int f, onc_u32 local,
-                                                 int sbx, int sbz) {
+__device__ __forceinline__ void onc_stencil_vec4(const onc_kargs& a, const onc_dsim& S, const onc_dfield& fd, int f,
+                                                 onc_u32 b, onc_u32 local, int sbx, int sbz) {
     const int X = S.dims[0], Y = S.dims[1], Z = S.dims[2];
     const int X4 = X >> 2;
     const int tw = onc_stencil_tw(X4, sbx < NT ? sbx : NT);
@@ -1060,21 +1104,98 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
     const int t = (int)threadIdx.x;
     const int x4 = tx * tw + t % tw;
     const int y = ty * th + t / tw;
-    // lanes past the row end or the last row leave; a remaining lane only
-    // takes a shuffled halo from a neighbour inside the row (hxm / hxp)
-    if (x4 >= X4 || y >= Y) return;
-    const int sl = t % W;  // lane inside its row segment
-    const int x0 = x4 * 4;
     const int z0 = tz * sbz;
     const int z1 = min(Z, z0 + sbz);
     const onc_u32 par = S.par;
     const float* __restrict__ in = S.fld[par][f];
     float* __restrict__ out = S.fld[par ^ 1u][f];
     const size_t XY = (size_t)X * (size_t)Y;
+    const bool dir = fd.bc == 0, psrc = fd.bc == 1;
+    onc_u32 ONC_GLOBAL* qout = nullptr;
+    if (a.quiet) {
+        const onc_u32 ONC_GLOBAL* qin = a.quiet + 2u * ((size_t)par * a.nqb);
+        qout = a.quiet + 2u * ((size_t)(par ^ 1u) * a.nqb);
+        const onc_u32 vb = qin[2u * b + 1u];
+        bool skip = qin[2u * b] != ONC_Q_NONE;
+        const onc_u32 sx = 1u, sy = (onc_u32)ntx, sz = (onc_u32)ntx * (onc_u32)nty;
+        const int ntz = (Z + sbz - 1) / sbz;
+        auto same = [&](onc_u32 nb) { return qin[2u * nb] != ONC_Q_NONE && qin[2u * nb + 1u] == vb; };
+        if (skip && tx > 0) skip = same(b - sx);
+        if (skip && tx + 1 < ntx) skip = same(b + sx);
+        if (skip && ty > 0) skip = same(b - sy);
+        if (skip && ty + 1 < nty) skip = same(b + sy);
+        if (skip && tz > 0) skip = same(b - sz);
+        if (skip && tz + 1 < ntz) skip = same(b + sz);
+        const float v = __uint_as_float(vb);
+        if (skip) skip = __float_as_uint(onc_sweep1(fd, v, v, v, v, v, v, v)) == vb;
+        if (skip && dir) skip = __float_as_uint(fd.bc_value) == vb;
+        if (skip && psrc && z0 == 0) skip = false;
+        bool kept = qout[2u * b] != ONC_Q_NONE && qout[2u * b + 1u] == vb;
+        if (skip) {
+            // The flags describe the buffers as the sweeps wrote them. The
+            // uptake of this step changed the input at the cubes stamped
+            // step + 1, that of the previous step the output at those stamped
+            // step: the tile and its one-voxel halo must hold none of the
+            // first, and the output tile none of the second to be kept.
+            const int xa = max(tx * tw * 4 - 1, 0), xb = min((tx + 1) * tw * 4, X - 1);
+            const int ya = max(ty * th - 1, 0), yb = min((ty + 1) * th, Y - 1);
+            const int za = max(z0 - 1, 0), zb = min(z1, Z - 1);
+            constexpr int QC = ONC_QCUBE;
+            const int ncx = (xb >> QC) - (xa >> QC) + 1, ncy = (yb >> QC) - (ya >> QC) + 1, ncz = (zb >> QC) - (za >> QC) + 1;
+            const onc_u32 now = S.step + 1u;
+            bool din = false, dout = false;
+            for (int c = t; c < ncx * ncy * ncz; c += NT) {
+                const int cx = (xa >> QC) + c % ncx, cy = (ya >> QC) + (c / ncx) % ncy, cz = (za >> QC) + c / (ncx * ncy);
+                const onc_u32 st = S.qstamp[onc_morton((onc_u32)cx, (onc_u32)cy, (onc_u32)cz, S.key_format)];
+                din = din || st == now;
+                dout = dout || st == S.step;
+            }
+            if (NT <= 64) {
+                din = __ballot(din) != 0ull;
+                dout = __ballot(dout) != 0ull;
+            } else {
+                din = __syncthreads_or(din);
+                dout = __syncthreads_or(dout);
+            }
+            skip = !din;
+            kept = kept && !dout;
+        }
+        if (skip) {
+            const bool have = kept;
+            if (!have && x4 < X4 && y < Y) {
+                const float4 o = make_float4(v, v, v, v);
+                size_t w = (size_t)x4 * 4u + (size_t)X * (size_t)y + XY * (size_t)z0;
+                for (int z = z0; z < z1; z++, w += XY) onc_st_nt((float4*)(out + w), o);
+            }
+            if (t == 0) {
+                qout[2u * b] = have ? ONC_Q_KEPT : ONC_Q_FILLED;
+                qout[2u * b + 1u] = vb;
+            }
+            return;
+        }
+    }
+    // Uniformity of the computed tile (NT = 256: four waves combine through
+    // LDS atomics; set up before any lane leaves)
+    __shared__ onc_u32 s_q[4];  // AND of the waves' verdicts, min and max first output, waves done
+    if (NT > 64 && qout) {
+        if (t == 0) {
+            s_q[0] = 1u;
+            s_q[1] = 0xFFFFFFFFu;
+            s_q[2] = 0u;
+            s_q[3] = 0u;
+        }
+        __syncthreads();
+    }
+    // lanes past the row end or the last row leave; a remaining lane only
+    // takes a shuffled halo from a neighbour inside the row (hxm / hxp)
+    if (x4 >= X4 || y >= Y) return;
+    const int sl = t % W;  // lane inside its row segment
+    const int x0 = x4 * 4;
     const size_t row = (size_t)x0 + (size_t)X * (size_t)y;
     const bool hxm = x0 > 0, hxp = x0 + 4 < X, hym = y > 0, hyp = y < Y - 1;
     const bool ldl = hxm && sl == 0, ldr = hxp && sl == W - 1;
-    const bool dir = fd.bc == 0, psrc = fd.bc == 1;
+    onc_u32 first = 0u;  // this lane's first output
+    bool uni = true;     // every output of this lane equals it
     // The centre column runs two planes ahead of the plane being computed (its
     // z+1 neighbour must already be in registers), the y rows and x halos one
     // plane ahead: every load has a whole iteration to arrive.
@@ -1127,6 +1248,12 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
         // the next sweep reads this plane from HBM anyway: a streaming
         // store keeps it from evicting the planes this sweep still reads
         onc_st_nt((float4*)(out + v), o);
+        if (qout) {
+            const onc_u32 ox = __float_as_uint(o.x);
+            if (z == z0) first = ox;
+            uni = uni && ox == first && __float_as_uint(o.y) == first && __float_as_uint(o.z) == first &&
+                  __float_as_uint(o.w) == first;
+        }
         cm = c;
         c = cp;
         cp = cpp;
@@ -1134,6 +1261,29 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_dsim& S, const onc_df
         yp = nyp;
         lx = nlx;
         rx = nrx;
+    }
+    if (!qout) return;
+    // the block's verdict: every lane uniform, all with the same first output
+    const onc_u32 f0 = onc_uniform(first);
+    const bool wave_ok = __ballot(!uni || first != f0) == 0ull;
+    if (NT <= 64) {
+        if (t == 0) {
+            qout[2u * b] = wave_ok ? ONC_Q_COMPUTED : ONC_Q_NONE;
+            qout[2u * b + 1u] = wave_ok ? f0 : 0u;
+        }
+        return;
+    }
+    // the last live wave to finish writes the verdict
+    if ((t & 63) == 0) {
+        if (!wave_ok) atomicAnd(&s_q[0], 0u);
+        atomicMin(&s_q[1], f0);
+        atomicMax(&s_q[2], f0);
+        const onc_u32 live = onc_stencil_live_waves(tw, th, X4 - tx * tw, Y - ty * th, NT);
+        if (atomicAdd(&s_q[3], 1u) + 1u == live) {
+            const bool ok = s_q[0] != 0u && s_q[1] == s_q[2];
+            qout[2u * b] = ok ? ONC_Q_COMPUTED : ONC_Q_NONE;
+            qout[2u * b + 1u] = ok ? s_q[1] : 0u;
+        }
     }
 }
 
@@ -1152,7 +1302,7 @@ __device__ __forceinline__ void onc_stencil_block(const onc_kargs& a, onc_u32 b)
     const onc_u32 local = b - a.jobs_prefix[jb];
     if ((X & 3) == 0) {
         const onc_dfield fdv = S.fields[f];
-        onc_stencil_vec4<NT>(S, fdv, f, local, a.sbx, sbz);
+        onc_stencil_vec4<NT>(a, S, fdv, f, b, local, a.sbx, sbz);
         return;
     }
     const int sbx = a.sbx < NT ? a.sbx : NT, sby = NT / sbx;

@@ -120,7 +120,8 @@ struct onc_batch {
     onc_u32* d_sort_status = nullptr;
     onc_u64* d_comp_status = nullptr;
     onc_u32* d_jobs = nullptr;
-    size_t cap_sims = 0, cap_tiles = 0, cap_ctiles = 0, cap_jobs = 0;
+    onc_u32* d_quiet = nullptr;  // quiet sweep tiles: {state, value} per block and field buffer
+    size_t cap_sims = 0, cap_tiles = 0, cap_ctiles = 0, cap_jobs = 0, cap_quiet = 0;
     onc_kargs ka{};
     onc_rule_args ra{};
     onc_u32 stencil_blocks = 0;
@@ -419,6 +420,7 @@ static void sim_layout(const onc_sim_desc* d, uint32_t* table, int* key_bits, si
     b += 2ull * d->n_fields * (*nvox) * 4;
     b += 4ull * (uint64_t)(*table) * 4;  // bins {start, end, epoch, 0}
     b += 2ull * d->n_chem_cols * chem_voxels(d) * 4;  // chemistry grid, two buffers
+    b += 4ull * ((maxkey >> (3 * ONC_QCUBE)) + 1u);   // uptake stamps of the sweep's quiet tiles
     b += 256;
     *bytes = b;
 }
@@ -513,6 +515,9 @@ extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) 
     }
     if ((rc = sim_alloc(s, 4ull * s->table * 4, &p)) != ONC_OK) return cleanup(rc);
     s->dsim.bins = (onc_u32*)p;
+    const size_t nstamps = (size_t)((s->table - 1u) >> (3 * ONC_QCUBE)) + 1u;
+    if ((rc = sim_alloc(s, 4ull * nstamps, &p)) != ONC_OK) return cleanup(rc);
+    s->dsim.qstamp = (onc_u32*)p;
     s->nchem = chem_voxels(d);
     void* chem_base = nullptr;
     if (d->n_chem_cols > 0) {
@@ -530,6 +535,7 @@ extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) 
     z.stop = 0xFFFFFFFFu;
     hipError_t e = hipMemcpyAsync(s->dstate, &z, sizeof(z), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = onc_launch_fill_u32(s->dsim.bins, 0xFFFFFFFFu, 4ull * s->table, st);
+    if (e == hipSuccess) e = onc_launch_fill_u32(s->dsim.qstamp, 0u, nstamps, st);
     if (e == hipSuccess && chem_base)
         e = onc_launch_fill_u32((onc_u32*)chem_base, 0u, 2ull * d->n_chem_cols * s->nchem, st);
     for (int f = 0; f < d->n_fields && e == hipSuccess; f++) {
@@ -734,6 +740,11 @@ extern "C" int onc_sim_upload_field(onc_sim* s, int f, const float* host) {
     onc_dstate st;
     CKR(sim_read_state(s, &st));
     CK(hipMemcpy(s->dsim.fld[st.parity][f], host, s->nvox * 4, hipMemcpyHostToDevice));
+    // the batch's record of uniform sweep tiles no longer holds
+    if (s->owner && s->owner->d_quiet) {
+        CK(hipMemsetAsync(s->owner->d_quiet, 0, s->owner->cap_quiet * 4, s->owner->stream));
+        CK(hipStreamSynchronize(s->owner->stream));
+    }
     return ONC_OK;
 }
 
@@ -803,6 +814,12 @@ extern "C" int onc_sim_set_step(onc_sim* s, uint32_t step) {
     }
     st.step = step;
     CK(hipMemcpy(s->dstate, &st, sizeof(st), hipMemcpyHostToDevice));
+    // the uptake stamps count steps: the batch's record of uniform sweep
+    // tiles no longer lines up with them
+    if (s->owner && s->owner->d_quiet) {
+        CK(hipMemsetAsync(s->owner->d_quiet, 0, s->owner->cap_quiet * 4, s->owner->stream));
+        CK(hipStreamSynchronize(s->owner->stream));
+    }
     return ONC_OK;
 }
 
@@ -844,6 +861,9 @@ static void batch_free_tables(onc_batch* b) {
     hipFree(b->d_sort_status);
     hipFree(b->d_comp_status);
     hipFree(b->d_jobs);
+    hipFree(b->d_quiet);
+    b->d_quiet = nullptr;
+    b->cap_quiet = 0;
     b->d_sims = nullptr;
     b->d_small = nullptr;
     b->d_hist = nullptr;
@@ -893,6 +913,8 @@ static int batch_rebuild(onc_batch* b) {
     std::vector<onc_dsim> host(S);
     std::vector<onc_u32> jobs, jprefix;
     onc_u32 blocks = 0;
+    // quiet sweep tiles, unless a sim's tissue rules also write the fields
+    bool quiet = true;
     for (size_t i = 0; i < S; i++) {
         onc_sim* s = b->sims[i];
         host[i] = s->dsim;
@@ -902,6 +924,8 @@ static int batch_rebuild(onc_batch* b) {
         total_cap += s->desc.capacity;
         const int X = s->desc.dims[0], Y = s->desc.dims[1], Z = s->desc.dims[2];
         const onc_u32 nb = stencil_blocks(X, Y, Z, b->sbx, b->sbz, b->fused ? 64 : 256);
+        host[i].job0 = (onc_u32)jobs.size();
+        if (s->tissue_passes) quiet = false;
         for (int f = 0; f < s->desc.n_fields; f++) {
             jobs.push_back((onc_u32)(i << 8) | (onc_u32)f);
             jprefix.push_back(blocks);
@@ -940,6 +964,17 @@ static int batch_rebuild(onc_batch* b) {
         b->cap_jobs = cj;
         CK(hipMemsetAsync(b->d_small, 0, (3 * (cs + 1) + ONC_NCOUNTERS + 16) * 4, b->stream));
     }
+    if ((size_t)blocks * 4 > b->cap_quiet) {
+        hipFree(b->d_quiet);
+        b->d_quiet = nullptr;
+        b->cap_quiet = 0;
+        void* p;
+        if (hipMalloc(&p, (size_t)blocks * 16) != hipSuccess) return fail(ONC_ERR_NOMEM, "quiet tiles");
+        b->d_quiet = (onc_u32*)p;
+        b->cap_quiet = (size_t)blocks * 4;
+    }
+    // every tile starts as not known to be uniform (the tiling may have changed)
+    if (b->d_quiet) CK(hipMemsetAsync(b->d_quiet, 0, b->cap_quiet * 4, b->stream));
     if (S) CK(hipMemcpyAsync(b->d_sims, host.data(), S * sizeof(onc_dsim), hipMemcpyHostToDevice, b->stream));
     std::vector<onc_u32> jj(jobs);
     jj.resize(b->cap_jobs, 0);
@@ -955,6 +990,8 @@ static int batch_rebuild(onc_batch* b) {
     a.tile = (onc_u32)tile;
     a.max_tiles = (onc_u32)max_tiles;
     a.xcd_tiles = onc_sort_xcd_tiles(b->items, b->ctx->cus);
+    a.quiet = quiet && blocks ? b->d_quiet : nullptr;
+    a.nqb = blocks;
     a.ctile = (onc_u32)ctile;
     a.max_ctiles = (onc_u32)max_ctiles;
     a.tile_prefix = b->d_small;
