@@ -200,3 +200,53 @@ def test_set_step_backwards_then_new_state_matches_oracle(ctx):
         compare(30, sim.download_cells(), [sim.download_field(f) for f in range(d.n_fields)], half)
     finally:
         sim.close()
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_quiet_sweep_tiles_match_the_oracle(ctx, fused):
+    """The sweep skips tiles whose input neighbourhood holds one value that
+    the update leaves unchanged. Four fields exercise it on a 64^3 grid: O2
+    (Dirichlet 1.0, consumed by the cells), a uniform zero-flux field (never
+    changes, so its tiles are kept), a field at 0.5 with Dirichlet faces at
+    1.0 (the front moves inward tile by tile), and a decaying zero-flux field
+    (no fixed point: always swept). At step 20 field 2 is replaced through the
+    host (a uniform 0.75 with a block at 0.2), and the step counter is set to
+    its own value (both must drop what the sweep knew). Every column and field
+    is bit-identical to the oracle at steps 20 and 40, with graph replays and
+    eager steps, in one launch with the forces and as its own launch."""
+    from oncolytica_amd.hip._abi import BC_DIRICHLET, BC_NONE
+    w = W.plain_c1()
+    w.fields = [W.FieldSpec(rate=0.1, init=1.0, bc=BC_DIRICHLET, bc_value=1.0, uptake=0.01),
+                W.FieldSpec(rate=0.05, init=0.25, bc=BC_NONE),
+                W.FieldSpec(rate=0.1, init=0.5, bc=BC_DIRICHLET, bc_value=1.0),
+                W.FieldSpec(rate=0.1, decay=0.01, init=0.5, bc=BC_NONE)]
+    d = w.desc()
+    cols = W.initial_cells(w)
+    st = O.TumorState(d, cols)
+    sim = Sim(ctx, d)
+    b = Batch(ctx)
+    try:
+        b.set_fused(fused)
+        sim.upload_cells(cols)
+        b.add(sim)
+        b.step(10, graph=True)
+        b.step(10, graph=False)
+        b.sync()
+        st.run(20)
+        print(f"quiet tiles fused={fused}: step 20", flush=True)
+        compare(20, sim.download_cells(), [sim.download_field(f) for f in range(d.n_fields)], st)
+        new = np.full((64, 64, 64), 0.75, dtype=np.float32)
+        new[20:40, 8:30, 30:50] = np.float32(0.2)
+        new = new.reshape(-1)
+        sim.upload_field(2, new)
+        st.fields[2][:] = new
+        sim.set_step(20)
+        b.step(10, graph=True)
+        b.step(10, graph=False)
+        b.sync()
+        st.run(20)
+        print(f"quiet tiles fused={fused}: step 40", flush=True)
+        compare(40, sim.download_cells(), [sim.download_field(f) for f in range(d.n_fields)], st)
+    finally:
+        b.close()
+        sim.close()
