@@ -41,22 +41,26 @@ def _log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(phase: str, n_mean: float, nvox: int, n_fields: int, cell_bytes: int, occ_bins: float):
+def algorithmic_bytes(phase: str, n_mean: float, nvox: int, n_fields: int, cell_bytes: int, occ_bins: float,
+                      sweep_bytes: float = None):
     """Algorithmic HBM bytes of ONE launch of a phase (units: cells/voxels of
     the timed steps). Per-unit figures: Morton 20 B/cell; radix pass 16 B/cell;
     permute 2S + key 4 B/cell + 8 B per occupied bin; forces 12 B read + 12 B
-    written + 4 B key per cell; stencil 8 B/voxel/field; forces and stencil
-    in one launch: the sum of the two; rule 2S + 4F + 4 B flags per cell;
-    compaction 4 B flag + 8 B scan + 2S per cell; the rule inside compaction
-    2S + 4F per cell (state and fields read, survivors written)."""
+    written + 4 B key per cell; stencil 8 B per swept voxel (read and
+    written) + 4 B per voxel filled with its quiet tile's value + 0 per voxel
+    kept (`sweep_bytes`, measured; 8 B/voxel/field when not given); forces and
+    stencil in one launch: the sum of the two; rule 2S + 4F + 4 B flags per
+    cell; compaction 4 B flag + 8 B scan + 2S per cell; the rule inside
+    compaction 2S + 4F per cell (state and fields read, survivors written)."""
     S = cell_bytes
+    sw = 8.0 * nvox * n_fields if sweep_bytes is None else sweep_bytes
     return {
         "morton": 20.0 * n_mean,
         "sort": 16.0 * n_mean,
         "permute_bins": (2 * S + 4) * n_mean + 8.0 * occ_bins,
         "forces": 28.0 * n_mean,
-        "stencil": 8.0 * nvox * n_fields,
-        "forces_stencil": 28.0 * n_mean + 8.0 * nvox * n_fields,
+        "stencil": sw,
+        "forces_stencil": 28.0 * n_mean + sw,
         "rule_compact": (2 * S + 4 * n_fields) * n_mean,
         "rule": (2 * S + 4 * n_fields + 4) * n_mean,
         "compact": (4 + 8 + 2 * S) * n_mean,
@@ -945,16 +949,35 @@ def main():
     gbatch.close()
     gsim.close()
 
-    for tag, tb in (("", table), ("separate launches: ", table_sep)):
+    # how the sweep treated the field voxels over the same steps (quiet tiles
+    # are written with their value or left alone instead of read and written),
+    # in each launch shape: the sweep's algorithmic bytes per launch
+    def sweep_treatment(fused):
+        qs, qb = fresh(fused)
+        tot = [0, 0, 0]
+        for _ in range(args.steps):
+            qb.step(1, graph=False)
+            for k, v in enumerate(qb.sweep_voxels()):
+                tot[k] += v
+        qb.close()
+        qs.close()
+        m = [v / max(1, args.steps) for v in tot]
+        return {"swept_voxels": m[0], "filled_voxels": m[1], "kept_voxels": m[2],
+                "bytes": 8.0 * m[0] + 4.0 * m[1], "dense_bytes": 8.0 * w.nvox * len(w.fields)}
+    sweep_f = sweep_treatment(True)
+    sweep_s = sweep_treatment(False)
+    _log(f"sweep per step: {sweep_f['swept_voxels']:.0f} voxels swept, {sweep_f['filled_voxels']:.0f} filled, "
+         f"{sweep_f['kept_voxels']:.0f} kept of {w.nvox * len(w.fields)} (one launch with the forces)")
+    for tag, tb, swb in (("", table, sweep_f["bytes"]), ("separate launches: ", table_sep, sweep_s["bytes"])):
         for name, row in tb.items():
-            byts = algorithmic_bytes(name, n_mean, w.nvox, len(w.fields), W.CELL_BYTES, occ)
+            byts = algorithmic_bytes(name, n_mean, w.nvox, len(w.fields), W.CELL_BYTES, occ, swb)
             gbs = (byts / (row["avg_us"] * 1e-6) / 1e9) if byts else None
             row.update({"alg_bytes": byts, "GBps": gbs})
             _log(f"{tag}{name:14s} {row['total_ms']:9.2f} ms {row['launches']:6d} launches "
                  f"{row['avg_us']:9.2f} us/launch"
                  + (f" {gbs:8.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:5.1f}% of peak)" if gbs else ""))
     dom_avg = dom_us / max(1, dom_n)
-    dom_bytes = algorithmic_bytes(dom, n_mean, w.nvox, len(w.fields), W.CELL_BYTES, occ)
+    dom_bytes = algorithmic_bytes(dom, n_mean, w.nvox, len(w.fields), W.CELL_BYTES, occ, sweep_f["bytes"])
     dom_gbs = dom_bytes / (dom_avg * 1e-6) / 1e9
     _log(f"timed region: {value / 1e6:.0f} M cell-updates/s, dominant kernel {dom} {dom_avg:.2f} us/launch "
          f"{dom_gbs:.0f} GB/s; graph replay {(gcu1 - gcu0) / (g1 - g0) / 1e6:.0f} M cell-updates/s; "
@@ -982,6 +1005,7 @@ def main():
                            "frac": row["GBps"] / HBM_PEAK_GBS,
                            "bound": "hbm" if ph == "stencil" else "gather latency"}
     roof["parts"] = parts
+    roof["sweep_treatment"] = {"one_launch": sweep_f, "separate_launch": sweep_s}
 
     # the long run: the workload stepped --long-steps times from its initial
     # state as graph replays, the whole run timed (the population grows, then

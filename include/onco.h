@@ -256,6 +256,13 @@ int onc_batch_profile(onc_batch* b, int phase_mask);
 int onc_batch_profile_every(onc_batch* b, int phase_mask, int every);
 #define ONC_PHASES 10
 int onc_batch_phase_times(onc_batch* b, double* us_total, uint64_t* launches, int n);
+/* how the last step's 7-point sweep treated the batch's field voxels (after
+   committing the step): read and written (swept), written with a tile's one
+   value without reading (filled), or left as they were (kept). A tile is
+   filled or kept when its input neighbourhood holds one value that the update
+   leaves unchanged; the fields are the same bits either way. No counterpart in
+   the reference (its sweep touches every voxel); for measurement. */
+int onc_batch_sweep_voxels(onc_batch* b, uint64_t* swept, uint64_t* filled, uint64_t* kept);
 
 /* ---- pure kernels on device pointers (per-kernel parity and roofline) ---- */
 /* keys[i] = Morton key of clamp(floor(p_i / dx), 0, dims-1) (template.wgsl:174-186) */

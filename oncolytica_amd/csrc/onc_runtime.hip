@@ -121,6 +121,7 @@ struct onc_batch {
     onc_u64* d_comp_status = nullptr;
     onc_u32* d_jobs = nullptr;
     onc_u32* d_quiet = nullptr;  // quiet sweep tiles: {state, value} per block and field buffer
+    std::vector<onc_u32> jobs_h, jprefix_h;  // host copies of the sweep job table
     size_t cap_sims = 0, cap_tiles = 0, cap_ctiles = 0, cap_jobs = 0, cap_quiet = 0;
     onc_kargs ka{};
     onc_rule_args ra{};
@@ -976,6 +977,8 @@ static int batch_rebuild(onc_batch* b) {
     // every tile starts as not known to be uniform (the tiling may have changed)
     if (b->d_quiet) CK(hipMemsetAsync(b->d_quiet, 0, b->cap_quiet * 4, b->stream));
     if (S) CK(hipMemcpyAsync(b->d_sims, host.data(), S * sizeof(onc_dsim), hipMemcpyHostToDevice, b->stream));
+    b->jobs_h = jobs;
+    b->jprefix_h = jprefix;
     std::vector<onc_u32> jj(jobs);
     jj.resize(b->cap_jobs, 0);
     std::vector<onc_u32> jp(jprefix);
@@ -1421,6 +1424,47 @@ extern "C" int onc_batch_profile_every(onc_batch* b, int phase_mask, int every) 
     for (int i = 0; i < ONC_PHASES; i++) {
         b->phase_us[i] = 0.0;
         b->phase_n[i] = 0;
+    }
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_sweep_voxels(onc_batch* b, uint64_t* swept, uint64_t* filled, uint64_t* kept) {
+    if (!b || !swept || !filled || !kept) return fail(ONC_ERR_INVALID, "NULL argument");
+    CK(hipSetDevice(b->ctx->device));
+    CKR(batch_commit(b));
+    *swept = *filled = *kept = 0;
+    if (b->dirty) return ONC_OK;
+    const onc_kargs& a = b->ka;
+    std::vector<onc_u32> q;
+    if (a.quiet) {
+        q.resize(4ull * a.nqb);
+        CK(hipMemcpy(q.data(), b->d_quiet, q.size() * 4, hipMemcpyDeviceToHost));
+    }
+    const int nt = b->fused ? 64 : 256;
+    for (size_t j = 0; j < b->jobs_h.size(); j++) {
+        onc_sim* s = b->sims[b->jobs_h[j] >> 8];
+        onc_dstate st;
+        CK(hipMemcpy(&st, s->dstate, sizeof(st), hipMemcpyDeviceToHost));
+        if (st.step >= st.stop) continue;  // frozen: not swept
+        const int X = s->desc.dims[0], Y = s->desc.dims[1], Z = s->desc.dims[2];
+        if (!a.quiet || (X & 3) != 0) {
+            *swept += (uint64_t)X * Y * Z;
+            continue;
+        }
+        // the last sweep wrote the buffer that is current now
+        const onc_u32* qo = q.data() + 2ull * st.parity * a.nqb;
+        const int X4 = X >> 2, tw = onc_stencil_tw(X4, std::min(b->sbx, nt)), th = nt / tw;
+        const int ntx = (X4 + tw - 1) / tw, nty = (Y + th - 1) / th;
+        for (onc_u32 blk = b->jprefix_h[j]; blk < b->jprefix_h[j + 1]; blk++) {
+            const onc_u32 l = blk - b->jprefix_h[j];
+            const int tx = (int)(l % ntx), ty = (int)((l / ntx) % nty), tz = (int)(l / ((onc_u32)ntx * nty));
+            const uint64_t vox = (uint64_t)(4 * std::min(tw, X4 - tx * tw)) * std::min(th, Y - ty * th) *
+                                 std::min(b->sbz, Z - tz * b->sbz);
+            const onc_u32 state = qo[2ull * blk];
+            if (state == 2u) *filled += vox;
+            else if (state == 3u) *kept += vox;
+            else *swept += vox;
+        }
     }
     return ONC_OK;
 }

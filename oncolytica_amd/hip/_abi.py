@@ -131,6 +131,7 @@ _PROTOS = {
     "onc_batch_profile": (C.c_int, [_P, C.c_int]),
     "onc_batch_profile_every": (C.c_int, [_P, C.c_int, C.c_int]),
     "onc_batch_phase_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_int]),
+    "onc_batch_sweep_voxels": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "onc_morton_keys": (C.c_int, [_P, _P, _P, _P, _u32, _i32p, C.c_float, C.c_int, _P]),
     "onc_sort_pairs": (C.c_int, [_P, _P, _P, _u32, C.c_int, _P, _P]),
     "onc_forces": (C.c_int, [_P, C.POINTER(SimDesc), _P, _P, _P, _u32, _P, _P, _P]),

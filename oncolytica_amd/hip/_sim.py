@@ -226,6 +226,13 @@ class Batch:
         call("onc_batch_phase_times", self.h, us, nl, _abi.PHASES)
         return {_abi.PHASE_NAMES[i]: (us[i], int(nl[i])) for i in range(_abi.PHASES)}
 
+    def sweep_voxels(self) -> tuple[int, int, int]:
+        """(swept, filled, kept) field voxels of the last step's sweep: read
+        and written, written with their tile's one value, left as they were."""
+        v = [C.c_uint64() for _ in range(3)]
+        call("onc_batch_sweep_voxels", self.h, *[C.byref(x) for x in v])
+        return tuple(int(x.value) for x in v)
+
     def close(self) -> None:
         if self.h:
             _abi.lib().onc_batch_destroy(self.h)
