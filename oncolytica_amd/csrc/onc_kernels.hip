@@ -1131,12 +1131,13 @@ __device__ __forceinline__ void onc_stencil_vec4(const onc_kargs& a, const onc_d
         if (skip && dir) skip = __float_as_uint(fd.bc_value) == vb;
         if (skip && psrc && z0 == 0) skip = false;
         bool kept = qout[2u * b] != ONC_Q_NONE && qout[2u * b + 1u] == vb;
-        if (skip) {
+        if (skip && fd.uptake != 0.0f) {
             // The flags describe the buffers as the sweeps wrote them. The
             // uptake of this step changed the input at the cubes stamped
             // step + 1, that of the previous step the output at those stamped
             // step: the tile and its one-voxel halo must hold none of the
-            // first, and the output tile none of the second to be kept.
+            // first, and the output tile none of the second to be kept. (A
+            // field without uptake is never changed between sweeps.)
             const int xa = max(tx * tw * 4 - 1, 0), xb = min((tx + 1) * tw * 4, X - 1);
             const int ya = max(ty * th - 1, 0), yb = min((ty + 1) * th, Y - 1);
             const int za = max(z0 - 1, 0), zb = min(z1, Z - 1);
