@@ -72,6 +72,21 @@ struct onc_dstate {
     onc_u32 pad[4];
 };
 
+// Bin table entries. The step with `completed` steps done writes its
+// occupied bins with tag onc_bin_tag(completed) in [1, ONC_BIN_TAGS], never 0
+// or 0xFFFF (what a cleared or filled entry holds); lengths are 16-bit (a
+// voxel with more cells is reported as an error). A tag comes round again
+// after ONC_BIN_TAGS steps; the Morton kernel of every step refills one of
+// ONC_BIN_SLICES slices of the table, so each entry is refilled twice in that
+// span and a stale entry can never carry the current tag.
+#define ONC_BIN_TAGS 65534u
+#define ONC_BIN_SLICES 32767u
+__host__ __device__ inline onc_u32 onc_bin_tag(onc_u32 completed) { return completed % ONC_BIN_TAGS + 1u; }
+// Length of entry b in the step with tag `tag` (0: empty, stale or unwritten).
+__device__ __forceinline__ onc_u32 onc_bin_len(uint2 b, onc_u32 tag) {
+    return (b.y & 0xFFFFu) == tag ? (b.y >> 16) : 0u;
+}
+
 // One entry of the packed segment table: a sim's geometry, model constants
 // and device buffers.
 struct onc_dsim {
@@ -102,10 +117,12 @@ struct onc_dsim {
     onc_u32 ONC_GLOBAL* keys[2];
     onc_u32 ONC_GLOBAL* vals[2];
     onc_u32 ONC_GLOBAL* flags;
-    // Bin table indexed by Morton key, 16 B per entry {start, end, epoch, 0}.
-    // An entry is valid in the step whose epoch (completed steps + 1) it
-    // carries; only occupied bins are written, so nothing is cleared per step.
+    // Bin table indexed by Morton key, 8 B per entry {start, len << 16 | tag}
+    // (onc_bin_tag): an entry is valid in the step whose tag it carries; only
+    // occupied bins are written, and the Morton kernel refills one slice of
+    // the table per step, so stale entries never reach a repeated tag.
     onc_u32 ONC_GLOBAL* bins;
+    onc_u32 table;  // entries of the bin table: Morton(dims - 1) + 1
     float ONC_GLOBAL* pos4;  // sorted positions as {x, y, z, 0}, written with the permute
     // Coarse chemistry grid: cdims voxels per axis, n_chem 32-bit columns, two
     // buffers. Every chemistry-rule pass reads one buffer and writes the other;
@@ -505,14 +522,14 @@ __device__ __forceinline__ onc_u32 onc_field_index(const onc_dsim& S, float px, 
 // Iterate the cells of the 27 Moore bins around (px, py, pz) in
 // template.wgsl:11-21 order (x fastest), skipping out-of-range bins and the
 // cell `self`. `body(j)` receives the sorted index of each neighbour. The
-// nine bin entries of a z-plane are loaded together (independent 16-byte
-// loads) before any of them is walked.
+// nine bin entries of a z-plane are loaded together (independent 8-byte
+// loads) before any of them is walked; `tag` is the step's bin tag.
 template <class F>
-__device__ __forceinline__ void onc_for_neighbors(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
+__device__ __forceinline__ void onc_for_neighbors(const onc_dsim& S, onc_u32 tag, float px, float py, float pz,
                                                   onc_u32 self, F body) {
     const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
               vz = onc_vox1(pz, S.dx, S.dims[2]);
-    const uint4* bins = (const uint4*)S.bins;
+    const uint2* bins = (const uint2*)S.bins;
     const onc_nkeys nk = onc_neighbor_keys(onc_morton((onc_u32)vx, (onc_u32)vy, (onc_u32)vz, S.key_format),
                                            S.key_format);
     for (int oz = -1; oz <= 1; oz++) {
@@ -520,18 +537,18 @@ __device__ __forceinline__ void onc_for_neighbors(const onc_dsim& S, onc_u32 epo
         if (nz < 0 || nz >= S.dims[2]) continue;
         // a select, not nk.z[oz + 1]: a runtime index would put nk in scratch memory
         const onc_u32 zk = oz < 0 ? nk.z[0] : (oz == 0 ? nk.z[1] : nk.z[2]);
-        uint4 b[9];
+        uint2 b[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             const int ny = vy + q / 3 - 1, nx = vx + q % 3 - 1;
             const bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
-            b[q] = make_uint4(0u, 0u, 0u, 0u);
+            b[q] = make_uint2(0u, 0u);
             if (in) b[q] = bins[nk.x[q % 3] | nk.y[q / 3] | zk];
         }
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            if (b[q].z != epoch) continue;
-            for (onc_u32 j = b[q].x; j < b[q].y; j++) {
+            const onc_u32 len = onc_bin_len(b[q], tag);
+            for (onc_u32 j = b[q].x; j < b[q].x + len; j++) {
                 if (j == self) continue;
                 body(j);
             }
@@ -544,11 +561,11 @@ __device__ __forceinline__ void onc_for_neighbors(const onc_dsim& S, onc_u32 epo
 // loads are issued together and the pairs are then accumulated in order.
 // `pair(o, j)` receives the neighbour's packed position {x, y, z, _} and index.
 template <class F>
-__device__ __forceinline__ void onc_for_neighbor_pos4(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
+__device__ __forceinline__ void onc_for_neighbor_pos4(const onc_dsim& S, onc_u32 tag, float px, float py, float pz,
                                                       onc_u32 self, const float4* __restrict__ pos4, F pair) {
     const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
               vz = onc_vox1(pz, S.dx, S.dims[2]);
-    const uint4* __restrict__ bins = (const uint4*)S.bins;
+    const uint2* __restrict__ bins = (const uint2*)S.bins;
     const onc_nkeys nk = onc_neighbor_keys(onc_morton((onc_u32)vx, (onc_u32)vy, (onc_u32)vz, S.key_format),
                                            S.key_format);
     for (int oz = -1; oz <= 1; oz++) {
@@ -561,9 +578,9 @@ __device__ __forceinline__ void onc_for_neighbor_pos4(const onc_dsim& S, onc_u32
         for (int q = 0; q < 9; q++) {
             const int ny = vy + q / 3 - 1, nx = vx + q % 3 - 1;
             const bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
-            uint4 b = make_uint4(0u, 0u, 0u, 0u);
+            uint2 b = make_uint2(0u, 0u);
             if (in) b = bins[nk.x[q % 3] | nk.y[q / 3] | zk];
-            const onc_u32 c = b.z == epoch ? b.y - b.x : 0u;
+            const onc_u32 c = onc_bin_len(b, tag);
             st[q] = b.x;
             pre[q + 1] = pre[q] + c;
         }

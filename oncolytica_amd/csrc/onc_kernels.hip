@@ -344,6 +344,16 @@ __global__ __launch_bounds__(256) void k_morton(onc_kargs a) {
         for (onc_u32 i = gid; i < ts; i += nthreads) a.sort_status[(size_t)p * a.max_tiles * 256u + i] = 0u;
     const onc_u32 tc = a.ctile_prefix[a.nsims];
     for (onc_u32 i = gid; i < tc; i += nthreads) a.comp_status[i] = 0ull;
+    // one slice of every sim's bin table back to the fill value (no kernel
+    // reads the bins between the last step's rule and this step's permute)
+    for (int s = 0; s < a.nsims; s++) {
+        const onc_dsim& S = a.sims[s];
+        const onc_u32 len = (S.table + ONC_BIN_SLICES - 1u) / ONC_BIN_SLICES;
+        const onc_u32 b0 = (S.step % ONC_BIN_SLICES) * len;
+        const onc_u32 b1 = min(b0 + len, S.table);
+        for (onc_u32 i = b0 + gid; i < b1; i += nthreads)
+            ((uint2*)S.bins)[i] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+    }
 }
 
 // Histograms only (keys already in keys[0]); used by onc_sort_pairs. A block
@@ -763,8 +773,9 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
                 cnt = (onc_u32)(63 - __builtin_clzll(act) - L + 1);
                 while (i + cnt < n && keys[i + cnt] == k) cnt++;
             }
-            // the whole bin entry {start, end, epoch, 0} in one 16-byte store
-            ((uint4*)S.bins)[k] = make_uint4(i, i + cnt, S.step + 1u, 0u);
+            // the whole bin entry {start, len << 16 | tag} in one 8-byte store
+            // (a longer bin is a device error; its length is clamped)
+            ((uint2*)S.bins)[k] = make_uint2(i, (min(cnt, ONC_MAX_BIN_CELLS) << 16) | onc_bin_tag(S.step));
             if (cnt > ONC_MAX_BIN_CELLS) atomicOr(a.err, ONC_DERR_BIG_BIN);
             // uptake, once per occupied voxel, before the sweep
             const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
@@ -800,13 +811,13 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // reads LDS when it moves to the next range. `batch_end()` runs after every
 // batch of eight.
 template <class F, class G>
-__device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, float px, float py, float pz,
+__device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 tag, float px, float py, float pz,
                                             onc_u32 self, const float4* __restrict__ pos4, onc_u32 (*s_rs)[ONC_FORCES_BLOCK],
                                             unsigned short (*s_rl)[ONC_FORCES_BLOCK], F pair, G batch_end) {
     const int tid = threadIdx.x & (ONC_FORCES_BLOCK - 1);  // lane: LDS slots are per wave
     const int vx = onc_vox1(px, S.dx, S.dims[0]), vy = onc_vox1(py, S.dx, S.dims[1]),
               vz = onc_vox1(pz, S.dx, S.dims[2]);
-    const uint4* __restrict__ bins = (const uint4*)S.bins;
+    const uint2* __restrict__ bins = (const uint2*)S.bins;
     const onc_nkeys nk = onc_neighbor_keys(onc_morton((onc_u32)vx, (onc_u32)vy, (onc_u32)vz, S.key_format),
                                            S.key_format);
     for (int oz = -1; oz <= 1; oz++) {
@@ -814,21 +825,21 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 epoch, fl
         if (nz < 0 || nz >= S.dims[2]) continue;
         // a select, not nk.z[oz + 1]: a runtime index would put nk in scratch memory
         const onc_u32 zk = oz < 0 ? nk.z[0] : (oz == 0 ? nk.z[1] : nk.z[2]);
-        uint4 b[9];
+        uint2 b[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             const int ny = vy + q / 3 - 1, nx = vx + q % 3 - 1;
             const bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
-            b[q] = make_uint4(0u, 0u, 0u, 0u);
+            b[q] = make_uint2(0u, 0u);
             if (in) b[q] = bins[nk.x[q % 3] | nk.y[q / 3] | zk];
         }
         onc_u32 nr = 0;
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            if (b[q].z == epoch && b[q].y > b[q].x) {
+            const onc_u32 len = onc_bin_len(b[q], tag);
+            if (len > 0u) {
                 s_rs[nr][tid] = b[q].x;
-                // clamped: a longer bin (reported as an error) still ends the walk
-                s_rl[nr][tid] = (unsigned short)min(b[q].y - b[q].x, ONC_MAX_BIN_CELLS);
+                s_rl[nr][tid] = (unsigned short)len;
                 nr++;
             }
         }
@@ -890,7 +901,7 @@ __device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u
     const int tid = threadIdx.x & (ONC_FORCES_BLOCK - 1);  // lane: LDS slots are per wave
     const onc_dsim& S = a.sims[s];
     const onc_u32 par = S.par;
-    const onc_u32 epoch = S.step + 1u;
+    const onc_u32 tag = onc_bin_tag(S.step);
     const float4* __restrict__ p4 = (const float4*)S.pos4;
     const float4 me = p4[i];
     const float px = me.x, py = me.y, pz = me.z;
@@ -939,7 +950,7 @@ __device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u
     // once any lane of the wave holds more, every lane still walking
     // flushes together. A flush taken by one lane at a time would run
     // the wave through the pair law once per lane.
-    forces_walk(S, epoch, px, py, pz, i, p4, s_rs, s_rl,
+    forces_walk(S, tag, px, py, pz, i, p4, s_rs, s_rl,
                 [&](const float4& o, onc_u32 j) {
                     const float ddx = px - o.x;
                     const float ddy = py - o.y;
@@ -1018,17 +1029,17 @@ __global__ __launch_bounds__(256) void k_pack_keys(const float* __restrict__ px,
     }
 }
 
-// Bin entries {start, end, epoch, 0} of sorted keys: the first cell of every
+// Bin entries {start, len << 16 | tag} of sorted keys: the first cell of every
 // run of equal keys counts the run (bins longer than the force walk's limit
 // set *err).
-__global__ __launch_bounds__(256) void k_bins4(const onc_u32* __restrict__ keys, onc_u32 n, uint4* bins,
-                                               onc_u32 epoch, onc_u32* err) {
+__global__ __launch_bounds__(256) void k_bins8(const onc_u32* __restrict__ keys, onc_u32 n, uint2* bins,
+                                               onc_u32 tag, onc_u32* err) {
     for (onc_u32 i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
         const onc_u32 k = keys[i];
         if (i > 0u && keys[i - 1u] == k) continue;
         onc_u32 e = i + 1u;
         while (e < n && keys[e] == k) e++;
-        bins[k] = make_uint4(i, e, epoch, 0u);
+        bins[k] = make_uint2(i, (min(e - i, ONC_MAX_BIN_CELLS) << 16) | tag);
         if (e - i > ONC_MAX_BIN_CELLS) atomicOr(err, ONC_DERR_BIG_BIN);
     }
 }
@@ -1977,7 +1988,7 @@ hipError_t onc_launch_forces_raw(const onc_kargs& a, const float* px, const floa
                                  float* fx, float* fy, float* fz, hipEvent_t t0, hipEvent_t t1, hipStream_t s) {
     const unsigned g = (unsigned)std::min<size_t>((n + 255u) / 256u, 65535u);
     hipLaunchKernelGGL(k_pack_keys, dim3(g), dim3(256), 0, s, px, py, pz, n, S, (float4*)pos4, keys);
-    hipLaunchKernelGGL(k_bins4, dim3(g), dim3(256), 0, s, keys, n, (uint4*)bins, 1u, err);
+    hipLaunchKernelGGL(k_bins8, dim3(g), dim3(256), 0, s, keys, n, (uint2*)bins, onc_bin_tag(0u), err);
     // one wave per 64 cells, a multiple of 8 blocks (onc_for_chunks_xcd)
     const unsigned waves = (unsigned)((((size_t)n + 63u) / 64u + 7u) / 8u * 8u);
     if (t0) (void)hipEventRecord(t0, s);

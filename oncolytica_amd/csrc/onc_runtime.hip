@@ -419,7 +419,7 @@ static void sim_layout(const onc_sim_desc* d, uint32_t* table, int* key_bits, si
     b += 5ull * cap * 4;              // keys x2, vals x2, flags
     b += 16ull * cap;                 // packed sorted positions
     b += 2ull * d->n_fields * (*nvox) * 4;
-    b += 4ull * (uint64_t)(*table) * 4;  // bins {start, end, epoch, 0}
+    b += 8ull * (uint64_t)(*table);      // bins {start, len << 16 | tag}
     b += 2ull * d->n_chem_cols * chem_voxels(d) * 4;  // chemistry grid, two buffers
     b += 4ull * ((maxkey >> (3 * ONC_QCUBE)) + 1u);   // uptake stamps of the sweep's quiet tiles
     b += 256;
@@ -514,8 +514,9 @@ extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) 
             for (int f = 0; f < d->n_fields; f++)
                 s->dsim.fld[b][f] = (float*)p + ((size_t)b * d->n_fields + f) * s->nvox;
     }
-    if ((rc = sim_alloc(s, 4ull * s->table * 4, &p)) != ONC_OK) return cleanup(rc);
+    if ((rc = sim_alloc(s, 8ull * s->table, &p)) != ONC_OK) return cleanup(rc);
     s->dsim.bins = (onc_u32*)p;
+    s->dsim.table = s->table;
     const size_t nstamps = (size_t)((s->table - 1u) >> (3 * ONC_QCUBE)) + 1u;
     if ((rc = sim_alloc(s, 4ull * nstamps, &p)) != ONC_OK) return cleanup(rc);
     s->dsim.qstamp = (onc_u32*)p;
@@ -535,7 +536,7 @@ extern "C" int onc_sim_create(onc_ctx* c, const onc_sim_desc* d, onc_sim** out) 
     onc_dstate z{};
     z.stop = 0xFFFFFFFFu;
     hipError_t e = hipMemcpyAsync(s->dstate, &z, sizeof(z), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = onc_launch_fill_u32(s->dsim.bins, 0xFFFFFFFFu, 4ull * s->table, st);
+    if (e == hipSuccess) e = onc_launch_fill_u32(s->dsim.bins, 0xFFFFFFFFu, 2ull * s->table, st);
     if (e == hipSuccess) e = onc_launch_fill_u32(s->dsim.qstamp, 0u, nstamps, st);
     if (e == hipSuccess && chem_base)
         e = onc_launch_fill_u32((onc_u32*)chem_base, 0u, 2ull * d->n_chem_cols * s->nchem, st);
@@ -798,13 +799,14 @@ extern "C" int onc_sim_set_step(onc_sim* s, uint32_t step) {
     CK(hipSetDevice(s->ctx->device));
     onc_dstate st;
     CKR(sim_read_state(s, &st));
-    if (step < st.step) {
-        // A bin entry is valid while its epoch equals the running step + 1, and
-        // only occupied keys are rewritten each step. Entries left by the steps
-        // being undone carry epochs that the coming steps reuse, so they would
-        // read as valid bins of voxels the restored cells do not occupy: clear
-        // the table, as at creation.
-        hipError_t e = onc_launch_fill_u32(s->dsim.bins, 0xFFFFFFFFu, 4ull * s->table, s->ctx->stream);
+    if (step != st.step) {
+        // A bin entry is valid while its tag is the running step's, and only
+        // occupied keys are rewritten each step; the per-step refill of one
+        // slice of the table assumes the steps go one by one. Entries left by
+        // other steps can carry tags that the coming steps reuse, so they would
+        // read as valid bins of voxels the cells do not occupy: clear the
+        // table, as at creation.
+        hipError_t e = onc_launch_fill_u32(s->dsim.bins, 0xFFFFFFFFu, 2ull * s->table, s->ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
         if (e != hipSuccess) return fail(ONC_ERR_HIP, std::string("set_step: ") + hipGetErrorString(e));
     }
@@ -1526,7 +1528,7 @@ extern "C" int onc_forces(onc_ctx* c, const onc_sim_desc* d, const float* px, co
         return o;
     };
     const size_t o_sim = take(sizeof(onc_dsim)), o_small = take(256), o_pos = take((size_t)n * 16),
-                 o_keys = take((size_t)n * 4), o_bins = take((size_t)table * 16);
+                 o_keys = take((size_t)n * 4), o_bins = take((size_t)table * 8);
     void* base;
     CKR(ctx_scratch(c, off, &base));
     char* B = (char*)base;
@@ -1540,7 +1542,8 @@ extern "C" int onc_forces(onc_ctx* c, const onc_sim_desc* d, const float* px, co
     ds.k_adh = d->k_adh;
     ds.mobility = d->mobility;
     ds.max_disp = d->max_disp;
-    ds.step = 0;  // bin entries carry epoch step + 1 = 1
+    ds.step = 0;  // bin entries carry onc_bin_tag(0)
+    ds.table = table;
     ds.n_live = n;
     ds.pos4 = (float*)(B + o_pos);
     ds.bins = (onc_u32*)(B + o_bins);
@@ -1548,7 +1551,7 @@ extern "C" int onc_forces(onc_ctx* c, const onc_sim_desc* d, const float* px, co
     hipStream_t st = c->stream;
     CK(hipMemcpyAsync(B + o_sim, &ds, sizeof(ds), hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(B + o_small, small, sizeof(small), hipMemcpyHostToDevice, st));
-    CK(hipMemsetAsync(B + o_bins, 0, (size_t)table * 16, st));
+    CK(hipMemsetAsync(B + o_bins, 0, (size_t)table * 8, st));  // tag 0: no entry valid
     onc_kargs a{};
     a.sims = (const onc_dsim*)(B + o_sim);
     a.nsims = 1;

@@ -899,9 +899,10 @@ class Translator:
         self.indent += 1
         self.emit(f"const int _ny = _vy + {q} / 3 - 1, _nx = _vx + {q} % 3 - 1;")
         self.emit("if (_nx < 0 || _ny < 0 || _nx >= _S.dims[0] || _ny >= _S.dims[1]) continue;")
-        self.emit(f"const uint4 {b} = ((const uint4*)_S.bins)[_nk.x[{q} % 3] | _nk.y[{q} / 3] | _zk];")
-        self.emit(f"if ({b}.z != ctx.epoch) continue;")
-        self.emit(f"for (onc_u32 {j} = {b}.x; {j} < {b}.y; {j}++) {{")
+        self.emit(f"const uint2 {b} = ((const uint2*)_S.bins)[_nk.x[{q} % 3] | _nk.y[{q} / 3] | _zk];")
+        self.emit(f"const onc_u32 {b}_n = onc_bin_len({b}, ctx.bin_tag);")
+        self.emit(f"if ({b}_n == 0u) continue;")
+        self.emit(f"for (onc_u32 {j} = {b}.x; {j} < {b}.x + {b}_n; {j}++) {{")
         self.indent += 1
         self.emit(f"if ({j} == {sv}._idx) continue;")
         self.emit(f"{ctype(st)} {var} = {st.cname}_load(_S, _S.st[ctx.par ^ 1u], {j});")
@@ -981,11 +982,12 @@ class Translator:
                 self.emit("{")
                 self.indent += 1
                 closers = 1
-            self.emit("const uint4 _b = ((const uint4*)_S.bins)[onc_morton((onc_u32)_nx, (onc_u32)_ny, "
+            self.emit("const uint2 _b = ((const uint2*)_S.bins)[onc_morton((onc_u32)_nx, (onc_u32)_ny, "
                       "(onc_u32)_nz, _S.key_format)];")
-            self.emit("if (_b.z != ctx.epoch) continue;" if chem else "if (_b.z == ctx.epoch)")
+            self.emit("const onc_u32 _bn = onc_bin_len(_b, ctx.bin_tag);")
+            self.emit("if (_bn == 0u) continue;" if chem else "if (_bn != 0u)")
             j = self.fresh("j")
-            self.emit(f"for (onc_u32 {j} = _b.x; {j} < _b.y; {j}++) {{")
+            self.emit(f"for (onc_u32 {j} = _b.x; {j} < _b.x + _bn; {j}++) {{")
             self.indent += 1
             self.emit(f"{ctype(vt)} {var} = {vt.cname}_load(_S, _S.st[ctx.par ^ 1u], {j});")
             closers += 1
@@ -1415,7 +1417,7 @@ class Translator:
             out.append(f"struct {tt.cname} {{\n{mem}\n}};\n")
         out.append(f"""struct RuleCtx {{
     const onc_dsim* S;
-    onc_u32 par, i, epoch;
+    onc_u32 par, i, bin_tag;
     onc_u32 tin;   // tissue field buffer read by tissue_at / voxel loads
     onc_u32 cin;   // chemistry buffer read by chemistry_at / voxel loads
     bool alive, div;
@@ -1506,7 +1508,7 @@ class Translator:
         ctx.S = &S;
         ctx.par = S.par;
         ctx.i = i;
-        ctx.epoch = S.step + 1u;
+        ctx.bin_tag = onc_bin_tag(S.step);
         ctx.tin = ctx.par ^ 1u;
         ctx.cin = (S.step * S.chem_passes + S.chem_passes) & 1u;
         ctx.alive = true;
@@ -1552,7 +1554,7 @@ class Translator:
     ctx.S = &S;
     ctx.par = S.par;
     ctx.i = 0xFFFFFFFFu;
-    ctx.epoch = S.step + 1u;
+    ctx.bin_tag = onc_bin_tag(S.step);
     ctx.tin = S.par ^ 1u ^ (a.pass & 1u);
     ctx.cin = (S.step * S.chem_passes) & 1u;
     ctx.alive = true;
@@ -1583,7 +1585,7 @@ class Translator:
     ctx.S = &S;
     ctx.par = S.par;
     ctx.i = 0xFFFFFFFFu;
-    ctx.epoch = S.step + 1u;
+    ctx.bin_tag = onc_bin_tag(S.step);
     ctx.tin = S.par ^ 1u;
     ctx.cin = (S.step * S.chem_passes + a.pass) & 1u;
     ctx.alive = true;
@@ -1660,7 +1662,7 @@ extern "C" __global__ __launch_bounds__(256) void onc_metric_main(onc_metric_arg
         ctx.S = &S;
         ctx.par = a.parity ^ 1u;
         ctx.i = i;
-        ctx.epoch = 0u;
+        ctx.bin_tag = 0u;
         ctx.tin = a.parity;
         ctx.cin = 0u;
         ctx.alive = true;

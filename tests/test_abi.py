@@ -57,10 +57,10 @@ def test_footprint_and_validation_without_device():
     b = _abi.footprint(d)
     # two state buffers + daughters (4 x 6 cols x 4 B), sort keys/vals/flags, packed
     # float4 positions, two
-    # fields double-buffered, a 2^24-entry {start, end, epoch, 0} bin table,
+    # fields double-buffered, a 2^24-entry {start, len << 16 | tag} bin table,
     # one 4-byte uptake stamp per 8^3 cube (2^24 / 512 of them)
     cap = 2_000_000
-    expect = (4 * 6 * cap * 4 + 5 * cap * 4 + 16 * cap + 2 * 2 * 256 ** 3 * 4 + 4 * 2 ** 24 * 4 + 4 * 2 ** 15
+    expect = (4 * 6 * cap * 4 + 5 * cap * 4 + 16 * cap + 2 * 2 * 256 ** 3 * 4 + 8 * 2 ** 24 + 4 * 2 ** 15
               + 256)
     assert b == expect
     bad = W.plain_c1().desc()

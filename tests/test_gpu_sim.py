@@ -172,7 +172,7 @@ def test_set_step_backwards_then_new_state_matches_oracle(ctx):
     """Run 30 steps, set the step counter back to 10 and upload the step-10
     state without its cells at x < 32: the next 20 steps equal the oracle's
     from that state. Bin entries written by the first run's steps 10..29
-    carry the epochs the next 20 steps use, and those of the removed half lie
+    carry the tags the next 20 steps use, and those of the removed half lie
     in the neighbourhoods of the cells left at x ~ 32: they must read as empty."""
     w = W.plain_c1()
     d = w.desc()
@@ -198,6 +198,35 @@ def test_set_step_backwards_then_new_state_matches_oracle(ctx):
         sim.sync()
         half.run(20)
         compare(30, sim.download_cells(), [sim.download_field(f) for f in range(d.n_fields)], half)
+    finally:
+        sim.close()
+
+
+def test_steps_across_the_bin_tag_wrap_match_oracle(ctx):
+    """Bin entries carry a 16-bit tag, step mod 65534 + 1. The step-10 state
+    of the 4,096-cell workload, restored at step 65,520, runs 30 steps across
+    the wrap (the step with 65,534 completed takes tag 1 again, after tag
+    65,534) bit for bit like the oracle from the same state (the step count
+    keys the Philox draws, so the oracle runs those same steps)."""
+    w = W.plain_c1()
+    d = w.desc()
+    ref = O.TumorState(d, W.initial_cells(w))
+    ref.run(10)
+    st = O.TumorState(d, ref.columns(), fields=ref.fields, next_id=ref.next_id.value)
+    st.step.value = 65_520
+    sim = Sim(ctx, d)
+    try:
+        sim.upload_cells(st.columns(), next_id=st.next_id.value)
+        for f in range(d.n_fields):
+            sim.upload_field(f, st.fields[f])
+        sim.set_step(65_520)
+        for k in (10, 20):
+            sim.step(k)
+            sim.sync()
+            st.run(k)
+            print(f"tag wrap: step {st.step.value}", flush=True)
+            compare(st.step.value, sim.download_cells(), [sim.download_field(f) for f in range(d.n_fields)], st)
+        assert st.step.value == 65_550
     finally:
         sim.close()
 
