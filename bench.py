@@ -404,13 +404,24 @@ def ensemble(ctx, n_sims: int, steps: int):
             "autotune_s": t_tune}
 
 
-def serve(ctx, rate: float, seconds: float, budget_gb: float):
+def serve(ctx, rate: float, seconds: float, budget_gb: float, progress_s: float = 0.0):
     """Served workload: Poisson arrivals, HBM-budget admission, continuous
-    batching; per-job latency = completion - arrival (wall clock)."""
+    batching; per-job latency = completion - arrival (wall clock). With
+    progress_s > 0 the server's admission log is printed at most once per
+    progress_s seconds (a long window then still shows progress)."""
     from oncolytica_amd.serving import Server, poisson_jobs
     jobs = poisson_jobs(rate, seconds)
     _log(f"serving: {len(jobs)} jobs over {seconds:.0f} s at {rate}/s, budget {budget_gb:.0f} GB")
-    srv = Server(ctx, hbm_budget=int(budget_gb * 1e9), log=None)
+    log = None
+    if progress_s > 0:
+        last = [-progress_s]
+
+        def log(msg):
+            now = time.perf_counter()
+            if now - last[0] >= progress_s:
+                last[0] = now
+                _log("serving: " + msg)
+    srv = Server(ctx, hbm_budget=int(budget_gb * 1e9), log=log)
     rep = srv.run(jobs)
     out = rep.summary()
     out.update({"workload": "served_poisson", "rate_jobs_per_s": rate, "arrival_window_s": seconds})
@@ -777,6 +788,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="arrival window of the served workload (Poisson, 4 jobs/s; 0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the served workload (--serve-seconds window) and print its record as the "
+                         "JSON line, with a progress line every 30 s")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-kernels", action="store_true",
                     help="skip the pure-kernel rooflines (sort, scan, sweep, Morton at large sizes)")
@@ -820,6 +834,18 @@ def main():
 
     if args.kernels_only:
         print(json.dumps({"kernels": kernel_rooflines(ctx)}))
+        return
+
+    if args.serve_only:
+        sv = serve(ctx, args.serve_rate, args.serve_seconds, args.serve_budget_gb, progress_s=30.0)
+        print(json.dumps({"metric": "p50/p99 job latency and cell-updates/s of the served workload",
+                          "value": sv["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
+                          "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": sv["workload"], "rate_jobs_per_s": sv["rate_jobs_per_s"],
+                                     "arrival_window_s": sv["arrival_window_s"],
+                                     "hbm_budget_bytes": sv["hbm_budget_bytes"], "device": info.name.decode(),
+                                     "cus": info.compute_units},
+                          "serving": sv}))
         return
 
     if args.stress_only:
