@@ -87,6 +87,11 @@ struct onc_module {
 
 struct onc_batch;
 
+// Steps per replay of a batch's multi-step graph, and the one-step replays
+// of the same batch after which it is captured.
+#define ONC_GRAPH_STEPS 16
+#define ONC_GRAPH_AFTER 64
+
 struct onc_sim {
     onc_ctx* ctx = nullptr;
     onc_sim_desc desc{};
@@ -137,9 +142,17 @@ struct onc_batch {
     int fuse_rule = 1;    // built-in rule fused into compaction where the schema allows
     bool rule_compact = false;  // this batch runs k_rule_compact (set by batch_rebuild)
     onc_u32 fs_groups = 0;  // groups of 8 blocks of that launch
-    // graph
+    // graphs: one step, and ONC_GRAPH_STEPS steps. A replay costs ~15 us of
+    // GPU time before its first kernel starts (eager launches pipeline at ~3
+    // us each), so a batch that keeps its members for a while replays the
+    // longer graph; it is captured once the one-step graph of the same batch
+    // has been replayed ONC_GRAPH_AFTER times (a capture of 16 steps costs
+    // host time that a batch rebuilt every few ticks would not win back).
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_n = nullptr;
+    hipGraphExec_t exec_n = nullptr;
+    uint64_t single_replays = 0;
     // profiling
     unsigned profile = 0;  // bit k: time phase k (onc_phase)
     bool prof_on = false;  // events are being recorded for the step being enqueued
@@ -853,11 +866,20 @@ static int batch_create_internal(onc_ctx* c, bool solo, onc_batch** out) {
     return ONC_OK;
 }
 
-static void batch_free_tables(onc_batch* b) {
+static void batch_drop_graphs(onc_batch* b) {
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec_n) hipGraphExecDestroy(b->exec_n);
+    if (b->graph_n) hipGraphDestroy(b->graph_n);
     b->exec = nullptr;
     b->graph = nullptr;
+    b->exec_n = nullptr;
+    b->graph_n = nullptr;
+    b->single_replays = 0;
+}
+
+static void batch_free_tables(onc_batch* b) {
+    batch_drop_graphs(b);
     hipFree(b->d_sims);
     hipFree(b->d_small);
     hipFree(b->d_hist);
@@ -939,10 +961,7 @@ static int batch_rebuild(onc_batch* b) {
     if (max_ctiles == 0) max_ctiles = 1;
     jprefix.push_back(blocks);
     // (re)allocate
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    batch_drop_graphs(b);
     if (S > b->cap_sims || max_tiles > b->cap_tiles || max_ctiles > b->cap_ctiles || jobs.size() + 1 > b->cap_jobs) {
         batch_free_tables(b);
         size_t cs = std::max<size_t>(S, 1), ct = max_tiles, cct = max_ctiles, cj = jobs.size() + 1;
@@ -1245,9 +1264,13 @@ static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
         }
         return ONC_OK;
     }
-    if (!b->exec) {
+    // Every step's work is decided on the device (the plan reads the live
+    // counts), so k steps captured back to back replay exactly like k
+    // replays of the one-step graph.
+    auto capture = [&](int steps, hipGraph_t* graph, hipGraphExec_t* exec) -> int {
         CK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(b);
+        int rc = ONC_OK;
+        for (int k = 0; k < steps && rc == ONC_OK; k++) rc = enqueue_step(b);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(b->stream, &g);
         if (rc != ONC_OK) {
@@ -1255,10 +1278,25 @@ static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
             return rc;
         }
         if (e != hipSuccess) return fail(ONC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        b->graph = g;
-        CK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        *graph = g;
+        CK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+        return ONC_OK;
+    };
+    for (int k = 0; k < nsteps;) {
+        if (nsteps - k >= ONC_GRAPH_STEPS) {
+            if (!b->exec_n && b->single_replays >= ONC_GRAPH_AFTER)
+                CKR(capture(ONC_GRAPH_STEPS, &b->graph_n, &b->exec_n));
+            if (b->exec_n) {
+                CK(hipGraphLaunch(b->exec_n, b->stream));
+                k += ONC_GRAPH_STEPS;
+                continue;
+            }
+        }
+        if (!b->exec) CKR(capture(1, &b->graph, &b->exec));
+        CK(hipGraphLaunch(b->exec, b->stream));
+        b->single_replays++;
+        k++;
     }
-    for (int k = 0; k < nsteps; k++) CK(hipGraphLaunch(b->exec, b->stream));
     return ONC_OK;
 }
 

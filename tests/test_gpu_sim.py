@@ -91,6 +91,10 @@ def test_capacity_clamp_matches_oracle(ctx):
 
 
 def test_deterministic_repeat_and_graph(ctx):
+    """Eager launches, graph replays and a second graph run give the same
+    bits over 110 steps. A batch replays its one-step graph 64 times before
+    it captures a 16-step graph, so the replays of 70 + 40 steps cover both
+    graphs, two replays of the 16-step one, and one-step replays after it."""
     w = W.plain_c1()
     outs = []
     for graph in (False, True, True):
@@ -98,7 +102,8 @@ def test_deterministic_repeat_and_graph(ctx):
         sim.upload_cells(W.initial_cells(w))
         b = Batch(ctx)
         b.add(sim)
-        b.step(30, graph=graph)
+        b.step(70, graph=graph)
+        b.step(40, graph=graph)
         b.sync()
         outs.append((sim.download_cells(), sim.download_field(0)))
         b.close()
