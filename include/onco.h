@@ -222,13 +222,26 @@ int onc_batch_destroy(onc_batch* b);
 int onc_batch_add(onc_batch* b, onc_sim* sim);
 int onc_batch_remove(onc_batch* b, onc_sim* sim);
 int onc_batch_size(onc_batch* b, int* n);
-/* use_graph: capture one timestep as a hipGraph and replay it nsteps times */
+/* use_graph: replay captured hipGraphs of the timestep (one step, and 16
+   steps once the one-step graph has been replayed 64 times) nsteps times */
 int onc_batch_step(onc_batch* b, int nsteps, int use_graph);
 int onc_batch_sync(onc_batch* b);
 /* tile shapes: cells per thread of a radix-sort tile and of a compaction
-   tile (4, 8, 12 or 16), and the stencil block (x, y with x*y == 256, z-chunk) */
+   tile (4, 8, 12 or 16), and the stencil block (x, y with x*y == 256, z-chunk).
+   The tile sizes given here are kept whatever the population (it sets the
+   tile policy's big_from to 0). */
 int onc_batch_set_tuning(onc_batch* b, int sort_items, int compact_items, int stencil_bx,
                          int stencil_by, int stencil_bz);
+/* Tile sizes by population (no counterpart in the reference, whose kernels
+   have one fixed workgroup size): a batch's steps use the tuned sizes until
+   its live cells reach big_from, then 16 / 8 cells per thread (sort /
+   compaction, where larger) until they fall below small_below. The live count
+   is read back asynchronously every 16 steps, so a change takes effect a few
+   steps late; results are the same bits either way. Default 786432 / 524288;
+   big_from = 0 keeps the tuned sizes. onc_batch_tile_items gives the sizes the
+   batch's next steps use. */
+int onc_batch_set_tile_policy(onc_batch* b, uint32_t big_from, uint32_t small_below);
+int onc_batch_tile_items(onc_batch* b, int* sort_items, int* compact_items);
 /* fused = 1 (the default): the neighbour forces and the 7-point sweep run as
    one launch, side by side on the same CUs (the force walk is bound by gather
    latency and ALU, the sweep by HBM); 0: two launches, forces then the sweep.

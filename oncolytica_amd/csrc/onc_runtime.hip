@@ -91,6 +91,14 @@ struct onc_batch;
 // of the same batch after which it is captured.
 #define ONC_GRAPH_STEPS 16
 #define ONC_GRAPH_AFTER 64
+// Default population thresholds of a batch's tile sizes (live cells), and
+// the sizes above them. On the 1M-cell workload 16 / 8 cells per thread are
+// 6-19 us per step faster than 8 / 4 from ~1M live cells up and 10-13 us
+// slower below ~0.5M.
+#define ONC_BIG_FROM 786432u
+#define ONC_SMALL_BELOW 524288u
+#define ONC_BIG_ITEMS 16
+#define ONC_BIG_CITEMS 8
 
 struct onc_sim {
     onc_ctx* ctx = nullptr;
@@ -134,8 +142,29 @@ struct onc_batch {
     uint64_t total_cap = 0;
     int passes = 1;
     // tuning
-    int items = 8;    // cells per thread of a sort tile
-    int citems = 4;   // cells per thread of a compaction tile
+    int items = 8;    // cells per thread of a sort tile (this step's)
+    int citems = 4;   // cells per thread of a compaction tile (this step's)
+    // Tile sizes by population: items / citems are the tuned sizes (below)
+    // until the batch's live cells reach big_from, then ONC_BIG_ITEMS /
+    // ONC_BIG_CITEMS (if larger) until they fall below small_below. Fewer,
+    // larger tiles shorten the look-back chains of a large population; small
+    // populations need small tiles to spread over the chip. The live count
+    // comes back from the device asynchronously every ONC_GRAPH_STEPS steps
+    // into one of two pinned slots; before a slot is reused the host waits for
+    // its last copy, so the host runs at most ~2 x ONC_GRAPH_STEPS steps ahead
+    // of the device and the count it acts on is that recent.
+    // big_from = 0: the tuned sizes always (onc_batch_set_tuning sets this).
+    int tuned_items = 8, tuned_citems = 4;
+    uint32_t big_from = ONC_BIG_FROM, small_below = ONC_SMALL_BELOW;
+    bool big = false;
+    onc_u32* h_live = nullptr;  // pinned: two slots of 16 words
+    hipEvent_t ev_live[2] = {nullptr, nullptr};
+    bool live_used[2] = {false, false};
+    uint64_t live_seq[2] = {0, 0};
+    uint64_t live_next_seq = 1, live_seen_seq = 0, live_decided_seq = 0;
+    uint32_t live_cells = 0;    // the newest count that came back
+    int live_slot = 0;          // the slot the next readback uses
+    int steps_since_readback = 0;
     int sbx = 64, sby = 4, sbz = 16;
     int fused = 1;    // forces and the sweep in one launch (k_forces_stencil)
     int force_waves = 0;  // cap on the one-wave blocks of the force walk (0: one per 64 cells of capacity)
@@ -903,6 +932,9 @@ static int batch_destroy_internal(onc_batch* b) {
     hipStreamSynchronize(b->stream);
     batch_free_tables(b);
     for (hipEvent_t e : b->ev) hipEventDestroy(e);
+    for (hipEvent_t e : b->ev_live)
+        if (e) hipEventDestroy(e);
+    if (b->h_live) hipHostFree(b->h_live);
     hipStreamDestroy(b->stream);
     delete b;
     return ONC_OK;
@@ -931,7 +963,23 @@ static int batch_rebuild(onc_batch* b) {
     CK(hipSetDevice(b->ctx->device));
     CK(hipStreamSynchronize(b->stream));
     const size_t S = b->sims.size();
-    const size_t tile = (size_t)256 * b->items, ctile = (size_t)256 * b->citems;
+    // tables sized for the tuned (smallest) tiles; the tile sizes of the
+    // coming steps follow the members' live cells
+    const size_t tile = (size_t)256 * b->tuned_items, ctile = (size_t)256 * b->tuned_citems;
+    // (read directly: the stream is idle, and sim_read_state would commit
+    // through this batch, whose rebuild is under way)
+    uint64_t live = 0;
+    for (onc_sim* s : b->sims) {
+        onc_dstate st;
+        CK(hipMemcpy(&st, s->dstate, sizeof(st), hipMemcpyDeviceToHost));
+        live += st.pending ? (uint64_t)st.nsurv + st.ndiv : st.n;
+    }
+    b->big = b->big_from != 0 && live >= b->big_from;
+    b->items = b->big ? std::max(b->tuned_items, ONC_BIG_ITEMS) : b->tuned_items;
+    b->citems = b->big ? std::max(b->tuned_citems, ONC_BIG_CITEMS) : b->tuned_citems;
+    b->live_used[0] = b->live_used[1] = false;  // the stream is idle: any copy has landed
+    b->live_seen_seq = b->live_decided_seq = b->live_next_seq - 1;
+    b->steps_since_readback = 0;
     size_t max_tiles = 0, max_ctiles = 0;
     int passes = 1;
     uint64_t total_cap = 0;
@@ -1011,12 +1059,12 @@ static int batch_rebuild(onc_batch* b) {
     a.sims = b->d_sims;
     a.nsims = (onc_u32)S;
     a.passes = (onc_u32)passes;
-    a.tile = (onc_u32)tile;
+    a.tile = (onc_u32)(256 * b->items);
     a.max_tiles = (onc_u32)max_tiles;
     a.xcd_tiles = onc_sort_xcd_tiles(b->items, b->ctx->cus);
     a.quiet = quiet && blocks ? b->d_quiet : nullptr;
     a.nqb = blocks;
-    a.ctile = (onc_u32)ctile;
+    a.ctile = (onc_u32)(256 * b->citems);
     a.max_ctiles = (onc_u32)max_ctiles;
     a.tile_prefix = b->d_small;
     a.cell_prefix = b->d_small + (b->cap_sims + 1);
@@ -1244,6 +1292,69 @@ static int check_rules(onc_batch* b) {
     return ONC_OK;
 }
 
+// Tile sizes of the coming steps from the newest live count that has come
+// back from the device (no wait here), and the next readback once
+// ONC_GRAPH_STEPS steps are enqueued (waiting, if need be, for the copy that
+// last used its slot). A change of sizes changes only launch arguments (the
+// tables are sized for the smallest tiles) and drops the captured graphs;
+// results do not depend on tile sizes.
+static void batch_take_live(onc_batch* b, int k) {
+    b->live_used[k] = false;
+    if (b->live_seq[k] > b->live_seen_seq) {
+        b->live_seen_seq = b->live_seq[k];
+        b->live_cells = b->h_live[16 * k];
+    }
+}
+
+static int batch_adapt_tiles(onc_batch* b) {
+    if (b->big_from == 0) return ONC_OK;
+    for (int k = 0; k < 2; k++) {
+        if (!b->live_used[k]) continue;
+        const hipError_t q = hipEventQuery(b->ev_live[k]);
+        if (q == hipErrorNotReady) continue;
+        CK(q);
+        batch_take_live(b, k);
+    }
+    // a count taken here or while batch_count_steps waited for a slot
+    if (b->live_seen_seq == b->live_decided_seq) return ONC_OK;
+    b->live_decided_seq = b->live_seen_seq;
+    const uint32_t live = b->live_cells;
+    const bool big = b->big ? live >= b->small_below : live >= b->big_from;
+    if (big == b->big) return ONC_OK;
+    b->big = big;
+    b->items = big ? std::max(b->tuned_items, ONC_BIG_ITEMS) : b->tuned_items;
+    b->citems = big ? std::max(b->tuned_citems, ONC_BIG_CITEMS) : b->tuned_citems;
+    b->ka.tile = (onc_u32)(256 * b->items);
+    b->ka.ctile = (onc_u32)(256 * b->citems);
+    b->ka.xcd_tiles = onc_sort_xcd_tiles(b->items, b->ctx->cus);
+    batch_drop_graphs(b);
+    return ONC_OK;
+}
+
+static int batch_count_steps(onc_batch* b, int steps) {
+    b->steps_since_readback += steps;
+    if (b->big_from == 0 || b->steps_since_readback < ONC_GRAPH_STEPS) return ONC_OK;
+    if (!b->h_live) {
+        void* p = nullptr;
+        CK(hipHostMalloc(&p, 2 * 16 * 4, hipHostMallocDefault));
+        b->h_live = (onc_u32*)p;
+        for (hipEvent_t& e : b->ev_live) CK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    const int k = b->live_slot;
+    if (b->live_used[k]) {
+        CK(hipEventSynchronize(b->ev_live[k]));
+        batch_take_live(b, k);
+    }
+    // the live cells of the last enqueued step (the plan's prefix total)
+    CK(hipMemcpyAsync(b->h_live + 16 * k, b->ka.cell_prefix + b->ka.nsims, 4, hipMemcpyDeviceToHost, b->stream));
+    CK(hipEventRecord(b->ev_live[k], b->stream));
+    b->live_used[k] = true;
+    b->live_seq[k] = b->live_next_seq++;
+    b->live_slot = k ^ 1;
+    b->steps_since_readback = 0;
+    return ONC_OK;
+}
+
 static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
     if (nsteps < 0) return fail(ONC_ERR_INVALID, "nsteps < 0");
     CK(hipSetDevice(b->ctx->device));
@@ -1254,6 +1365,7 @@ static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
     if (profiling) CKR(prof_reserve(b, nsteps));
     if (!use_graph) {
         for (int k = 0; k < nsteps; k++) {
+            if (k % ONC_GRAPH_STEPS == 0) CKR(batch_adapt_tiles(b));
             const bool timed = profiling && (b->prof_tick++ % (uint64_t)b->prof_every) == 0u;
             b->prof_on = timed;
             if (timed) b->ev_mask.push_back(0u);
@@ -1261,6 +1373,7 @@ static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
             if (timed) b->ev_used++;
             b->prof_on = false;
             CKR(rc);
+            CKR(batch_count_steps(b, 1));
         }
         return ONC_OK;
     }
@@ -1283,12 +1396,14 @@ static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
         return ONC_OK;
     };
     for (int k = 0; k < nsteps;) {
+        CKR(batch_adapt_tiles(b));
         if (nsteps - k >= ONC_GRAPH_STEPS) {
             if (!b->exec_n && b->single_replays >= ONC_GRAPH_AFTER)
                 CKR(capture(ONC_GRAPH_STEPS, &b->graph_n, &b->exec_n));
             if (b->exec_n) {
                 CK(hipGraphLaunch(b->exec_n, b->stream));
                 k += ONC_GRAPH_STEPS;
+                CKR(batch_count_steps(b, ONC_GRAPH_STEPS));
                 continue;
             }
         }
@@ -1296,6 +1411,7 @@ static int batch_step_impl(onc_batch* b, int nsteps, int use_graph) {
         CK(hipGraphLaunch(b->exec, b->stream));
         b->single_replays++;
         k++;
+        CKR(batch_count_steps(b, 1));
     }
     return ONC_OK;
 }
@@ -1406,12 +1522,32 @@ extern "C" int onc_batch_set_tuning(onc_batch* b, int items, int citems, int bx,
         return fail(ONC_ERR_INVALID, "stencil block must satisfy bx*by == 256, 1 <= bz <= 1024");
     CK(hipSetDevice(b->ctx->device));
     CK(hipStreamSynchronize(b->stream));
-    b->items = items;
-    b->citems = citems;
+    b->tuned_items = b->items = items;
+    b->tuned_citems = b->citems = citems;
+    b->big_from = 0;  // tuned sizes are kept whatever the population (onc_batch_set_tile_policy)
     b->sbx = bx;
     b->sby = by;
     b->sbz = bz;
     b->dirty = true;
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_set_tile_policy(onc_batch* b, uint32_t big_from, uint32_t small_below) {
+    if (!b) return fail(ONC_ERR_INVALID, "NULL batch");
+    if (big_from != 0 && (small_below == 0 || small_below > big_from))
+        return fail(ONC_ERR_INVALID, "tile policy needs 0 < small_below <= big_from (or big_from = 0)");
+    CK(hipSetDevice(b->ctx->device));
+    CK(hipStreamSynchronize(b->stream));
+    b->big_from = big_from;
+    b->small_below = small_below;
+    b->dirty = true;
+    return ONC_OK;
+}
+
+extern "C" int onc_batch_tile_items(onc_batch* b, int* items, int* citems) {
+    if (!b || !items || !citems) return fail(ONC_ERR_INVALID, "NULL argument");
+    *items = b->items;
+    *citems = b->citems;
     return ONC_OK;
 }
 

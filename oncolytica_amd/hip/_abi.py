@@ -125,6 +125,8 @@ _PROTOS = {
     "onc_batch_step": (C.c_int, [_P, C.c_int, C.c_int]),
     "onc_batch_sync": (C.c_int, [_P]),
     "onc_batch_set_tuning": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "onc_batch_set_tile_policy": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "onc_batch_tile_items": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "onc_batch_set_fused": (C.c_int, [_P, C.c_int]),
     "onc_batch_set_fused_rule": (C.c_int, [_P, C.c_int]),
     "onc_batch_set_forces_waves": (C.c_int, [_P, C.c_int]),

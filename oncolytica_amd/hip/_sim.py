@@ -184,6 +184,18 @@ class Batch:
     def set_tuning(self, items: int, citems: int, bx: int, by: int, bz: int) -> None:
         call("onc_batch_set_tuning", self.h, items, citems, bx, by, bz)
 
+    def set_tile_policy(self, big_from: int, small_below: int) -> None:
+        """Tile sizes by population: the tuned sizes below `big_from` live
+        cells, 16 / 8 cells per thread from there until the population falls
+        below `small_below` (big_from = 0: the tuned sizes always)."""
+        call("onc_batch_set_tile_policy", self.h, int(big_from), int(small_below))
+
+    def tile_items(self) -> tuple[int, int]:
+        """Cells per thread of the sort and compaction tiles of the next steps."""
+        a, c = C.c_int(), C.c_int()
+        call("onc_batch_tile_items", self.h, C.byref(a), C.byref(c))
+        return a.value, c.value
+
     def set_fused(self, fused: bool) -> None:
         """Forces and the 7-point sweep in one launch (True, the default) or
         in two (False); results are identical."""

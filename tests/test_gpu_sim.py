@@ -114,6 +114,42 @@ def test_deterministic_repeat_and_graph(ctx):
         np.testing.assert_array_equal(outs[0][1].view(np.uint32), f.view(np.uint32))
 
 
+def test_tile_sizes_follow_the_population_bit_exact(ctx):
+    """A batch switches to 16 / 8 cells per thread once its live cells reach
+    big_from and back below small_below, from live counts read back every 16
+    steps. The 4,096-cell workload grows past 9,000 cells by step 30, falls
+    to ~8,800 by step 40 and grows again: with thresholds 9,000 / 8,900 it
+    switches both ways, through eager steps and graph replays, and stays
+    bit-identical to the oracle (tile sizes never change results)."""
+    w = W.plain_c1()
+    d = w.desc()
+    cols = W.initial_cells(w)
+    st = O.TumorState(d, cols)
+    sim = Sim(ctx, d)
+    sim.upload_cells(cols)
+    b = Batch(ctx)
+    b.add(sim)
+    b.set_tile_policy(9000, 8900)
+    try:
+        assert b.tile_items() == (8, 4)
+        seen = set()
+        done = 0
+        for k, graph in [(20, False), (16, True), (8, False), (16, True), (20, False), (20, True)]:
+            b.step(k, graph=graph)
+            b.sync()
+            done += k
+            seen.add(b.tile_items())
+        st.run(done)
+        print(f"tile policy: {done} steps, sizes seen {sorted(seen)}", flush=True)
+        compare(done, sim.download_cells(), [sim.download_field(f) for f in range(d.n_fields)], st)
+        assert (16, 8) in seen and (8, 4) in seen
+        b.set_tile_policy(0, 0)
+        assert b.tile_items() == (8, 4) or b.tile_items() == (16, 8)
+    finally:
+        b.close()
+        sim.close()
+
+
 def test_batch_equals_alone(ctx):
     """Five ensemble members stepped as one batch (graph replays of 5 and 7
     steps) equal each member stepped alone, and each equals the oracle run
