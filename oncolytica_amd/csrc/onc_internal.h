@@ -56,7 +56,8 @@ enum onc_phase {
 #define ONC_PLAN_STEP 0
 #define ONC_PLAN_COMMIT 1
 hipError_t onc_launch_plan(const onc_kargs& a, int mode, hipStream_t s);
-hipError_t onc_launch_morton(const onc_kargs& a, int grid, hipStream_t s);
+// runs: count digits once per run of equal digits in a wave (large populations)
+hipError_t onc_launch_morton(const onc_kargs& a, int grid, bool runs, hipStream_t s);
 hipError_t onc_launch_hist_only(const onc_kargs& a, int items, int grid, hipStream_t s);
 hipError_t onc_launch_sort_pass(const onc_kargs& a, int items, int pass, int grid, hipStream_t s);
 // Blocks of the radix pass with `items` cells per thread that one CU holds at

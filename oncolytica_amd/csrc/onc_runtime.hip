@@ -148,7 +148,8 @@ struct onc_batch {
     // until the batch's live cells reach big_from, then ONC_BIG_ITEMS /
     // ONC_BIG_CITEMS (if larger) until they fall below small_below. Fewer,
     // larger tiles shorten the look-back chains of a large population; small
-    // populations need small tiles to spread over the chip. The live count
+    // populations need small tiles to spread over the chip. The same class
+    // picks the Morton kernel's run-length digit counting (big only). The live count
     // comes back from the device asynchronously every ONC_GRAPH_STEPS steps
     // into one of two pinned slots; before a slot is reused the host waits for
     // its last copy, so the host runs at most ~2 x ONC_GRAPH_STEPS steps ahead
@@ -1238,7 +1239,7 @@ static int enqueue_step(onc_batch* b) {
     CK(onc_launch_plan(a, ONC_PLAN_STEP, st));
     prof_mark(b, ONC_PH_PLAN, 1);
     prof_mark(b, ONC_PH_MORTON, 0);
-    CK(onc_launch_morton(a, grid_morton(b), st));
+    CK(onc_launch_morton(a, grid_morton(b), b->big, st));
     prof_mark(b, ONC_PH_MORTON, 1);
     prof_mark(b, ONC_PH_SORT, 0);
     for (int p = 0; p < b->passes; p++) CK(onc_launch_sort_pass(a, b->items, p, gt, st));
