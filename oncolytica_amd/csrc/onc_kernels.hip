@@ -837,11 +837,35 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // Candidates of the 27 Moore bins around (px, py, pz), z-plane by z-plane, in
 // template.wgsl:11-21 order (x fastest) and bin order inside a bin, skipping
 // `self`; `pair(o, j)` gets each candidate's packed position. Per plane the
-// nine bin entries are loaded together, the non-empty ranges are compacted
-// into the thread's LDS slots, and a cursor walks them eight candidates at a
-// time (the eight position loads are in flight together); the cursor only
-// reads LDS when it moves to the next range. `batch_end()` runs after every
-// batch of eight.
+// entries of the plane's reachable bins are loaded together, the non-empty
+// ranges are compacted into the thread's LDS slots, and a cursor walks them
+// ONC_WALK_BATCH candidates at a time (their position loads are in flight
+// together). `batch_end()` runs after every batch.
+//
+// A bin is reachable when the distance from (px, py, pz) to the nearest
+// point of its voxel can be below R: face neighbours always, edge and corner
+// neighbours only when the cell sits near that edge or corner (with dx = R,
+// 20.6 of the 27 bins on average). A candidate in an unreachable bin is at
+// least R from the cell, so the pair filter (d2 < R2) would reject it: the
+// gaps are taken short by a margin (2^-20 of the grid's longest side, far
+// above the rounding of the voxel assignment and of the gaps themselves) and
+// their squared sum must reach R2 (1 + 2^-18), above the rounding of d2.
+// Skipping such bins leaves every pair list, and so every force, unchanged.
+//
+// The cursor is branch-free: every slot reads the LDS range at the cursor's
+// (clamped) next index and selects it when the current range ends, and a
+// slot past the lane's last candidate names `self` (its load is a cached
+// re-read of the cell's own position; the filter skips it).
+#ifndef ONC_WALK_BATCH
+#define ONC_WALK_BATCH 4
+#endif
+#ifndef ONC_WALK_REACH
+#define ONC_WALK_REACH 1
+#endif
+#ifndef ONC_WALK_SELECT
+#define ONC_WALK_SELECT 1
+#endif
+static_assert(ONC_PAIR_LIST / 2 + ONC_WALK_BATCH <= ONC_PAIR_LIST, "a batch must fit the pair list");
 template <class F, class G>
 __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 tag, float px, float py, float pz,
                                             onc_u32 self, const float4* __restrict__ pos4, onc_u32 (*s_rs)[ONC_FORCES_BLOCK],
@@ -852,16 +876,42 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 tag, floa
     const uint2* __restrict__ bins = (const uint2*)S.bins;
     const onc_nkeys nk = onc_neighbor_keys(onc_morton((onc_u32)vx, (onc_u32)vy, (onc_u32)vz, S.key_format),
                                            S.key_format);
+#if ONC_WALK_REACH
+    // squared gaps to the lower (m) and upper (p) faces of the cell's voxel
+    const float dx = S.dx;
+    const float del = (float)max(S.dims[0], max(S.dims[1], S.dims[2])) * dx * 0x1p-20f;
+    auto lo2 = [&](float p, int v) {
+        const float g = (p - (float)v * dx) - del;
+        return g > 0.0f ? g * g : 0.0f;
+    };
+    auto hi2 = [&](float p, int v) {
+        const float g = ((float)(v + 1) * dx - p) - del;
+        return g > 0.0f ? g * g : 0.0f;
+    };
+    const float gxm = lo2(px, vx), gxp = hi2(px, vx);
+    const float gym = lo2(py, vy), gyp = hi2(py, vy);
+    const float gzm = lo2(pz, vz), gzp = hi2(pz, vz);
+    const float reach = (S.R * S.R) * (1.0f + 0x1p-18f);
+#endif
     for (int oz = -1; oz <= 1; oz++) {
         const int nz = vz + oz;
         if (nz < 0 || nz >= S.dims[2]) continue;
+#if ONC_WALK_REACH
+        const float gz = oz < 0 ? gzm : (oz == 0 ? 0.0f : gzp);
+        if (!(gz < reach)) continue;
+#endif
         // a select, not nk.z[oz + 1]: a runtime index would put nk in scratch memory
         const onc_u32 zk = oz < 0 ? nk.z[0] : (oz == 0 ? nk.z[1] : nk.z[2]);
         uint2 b[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             const int ny = vy + q / 3 - 1, nx = vx + q % 3 - 1;
-            const bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
+            bool in = nx >= 0 && ny >= 0 && nx < S.dims[0] && ny < S.dims[1];
+#if ONC_WALK_REACH
+            const float gy = q / 3 == 0 ? gym : (q / 3 == 1 ? 0.0f : gyp);
+            const float gx = q % 3 == 0 ? gxm : (q % 3 == 1 ? 0.0f : gxp);
+            in = in && (gz + gy) + gx < reach;
+#endif
             b[q] = make_uint2(0u, 0u);
             if (in) b[q] = bins[nk.x[q % 3] | nk.y[q / 3] | zk];
         }
@@ -881,10 +931,21 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 tag, floa
             e = j + s_rl[0][tid];
         }
         while (k < nr) {
-            onc_u32 jj[8];
+            onc_u32 jj[ONC_WALK_BATCH];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                jj[u] = k < nr ? j : 0xFFFFFFFFu;
+            for (int u = 0; u < ONC_WALK_BATCH; u++) {
+#if ONC_WALK_SELECT
+                const bool v = k < nr;
+                jj[u] = v ? j : self;
+                const onc_u32 j1 = j + 1u;
+                const bool end = v && j1 == e;
+                k += end ? 1u : 0u;
+                const onc_u32 kk = k < 8u ? k : 8u;
+                const onc_u32 ns = s_rs[kk][tid], nl = s_rl[kk][tid];
+                j = end ? ns : j1;
+                e = end ? ns + nl : e;
+#else
+                jj[u] = k < nr ? j : self;
                 if (k < nr) {
                     j++;
                     if (j == e) {
@@ -895,13 +956,14 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 tag, floa
                         }
                     }
                 }
+#endif
             }
-            float4 p[8];
+            float4 p[ONC_WALK_BATCH];
 #pragma unroll
-            for (int u = 0; u < 8; u++) p[u] = jj[u] != 0xFFFFFFFFu ? pos4[jj[u]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (int u = 0; u < ONC_WALK_BATCH; u++) p[u] = pos4[jj[u]];
 #pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (jj[u] != 0xFFFFFFFFu && jj[u] != self) pair(p[u], jj[u]);
+            for (int u = 0; u < ONC_WALK_BATCH; u++)
+                if (jj[u] != self) pair(p[u], jj[u]);
             batch_end();
         }
     }
@@ -978,7 +1040,7 @@ __device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u
     // Phase A: cheap distance filter over the 27 bins; only pairs inside
     // the interaction radius reach the expensive (sqrt, divide) path, so
     // lanes diverge on short lists instead of on every candidate. A list
-    // holds at most 8 entries between batches (a batch adds at most 8);
+    // holds at most 8 entries between batches (a batch adds at most ONC_WALK_BATCH);
     // once any lane of the wave holds more, every lane still walking
     // flushes together. A flush taken by one lane at a time would run
     // the wave through the pair law once per lane.
