@@ -260,6 +260,55 @@ def test_forces_bit_exact_with_oracle(ctx, case):
         assert max(np.abs(g).max() for g in got) > 0.0
 
 
+def _pairs_at_reach(d, n, rng):
+    """n pairs of cells straddling a face, edge or corner of a voxel, whose
+    distance is R (1 + e) with e in [-3e-6, 3e-6]: the first cell's gaps to
+    the neighbour voxel add up (in quadrature) to about R, the second sits on
+    or just past the shared boundary."""
+    dx, R = float(d.dx), float(d.interaction_radius)
+    X = d.dims[0]
+    offs = [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1) if (a, b, c) != (0, 0, 0)]
+    pts = []
+    for _ in range(n):
+        o = np.array(offs[rng.integers(len(offs))])
+        v = rng.integers(2, X - 2, size=3)
+        nz = np.nonzero(o)[0]
+        w = rng.random(len(nz)) + 0.2
+        g = np.zeros(3)
+        g[nz] = w / np.linalg.norm(w) * R * (1.0 + rng.uniform(-3e-6, 3e-6))
+        g = np.minimum(g, dx)
+        a = (v + rng.random(3)) * dx
+        b = a.copy()
+        past = rng.choice([0.0, 1e-7, 1e-6]) * dx
+        for ax in nz:
+            if o[ax] > 0:
+                a[ax] = (v[ax] + 1) * dx - g[ax]
+                b[ax] = (v[ax] + 1) * dx + past
+            else:
+                a[ax] = v[ax] * dx + g[ax]
+                b[ax] = v[ax] * dx - past
+        pts += [a, b]
+    return np.array(pts, dtype=np.float32)
+
+
+@pytest.mark.parametrize("dx", [1.0, 2.5])
+def test_forces_at_the_interaction_radius_bit_exact(ctx, dx):
+    """Pairs at R (1 +- 3e-6) across voxel faces, edges and corners, with
+    voxels as wide as R and wider: the force walk skips only bins out of reach
+    of the cell, so it finds every pair the oracle finds, bit for bit."""
+    from oncolytica_amd import workloads as Wl
+    d = Wl.plain_c1().desc()
+    d.dx = dx
+    d.k_adh = 0.2
+    rng = np.random.default_rng(int(dx * 10))
+    p = _pairs_at_reach(d, 6000, rng)
+    k = O.keys(p[:, 0], p[:, 1], p[:, 2], tuple(d.dims), d.dx, d.key_format)
+    p = np.ascontiguousarray(p[np.argsort(k, kind="stable")])
+    ref = O.forces_raw(d, p[:, 0], p[:, 1], p[:, 2], "f32")
+    assert (np.abs(np.stack(ref)).max(0) > 0).mean() > 0.3  # many pairs fall inside R
+    _forces_case(ctx, d, p)
+
+
 def test_pinned_host_buffers_round_trip(ctx):
     """onc_host_alloc: page-locked host memory moves data to and from the
     device like ordinary host memory."""
