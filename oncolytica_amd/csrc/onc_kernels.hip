@@ -835,8 +835,10 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 #define ONC_FORCES_BLOCK 64  // one wave per block
 
 // Candidates of the 27 Moore bins around (px, py, pz), z-plane by z-plane, in
-// template.wgsl:11-21 order (x fastest) and bin order inside a bin, skipping
-// `self`; `pair(o, j)` gets each candidate's packed position. Per plane the
+// template.wgsl:11-21 order (x fastest) and bin order inside a bin;
+// `pair(o, j)` gets each candidate's packed position, `self` included (and
+// named again by spent slots): its d2 is exactly 0, which the pair filter
+// rejects as it rejects any coincident cell. Per plane the
 // entries of the plane's reachable bins are loaded together, the non-empty
 // ranges are compacted into the thread's LDS slots, and a cursor walks them
 // ONC_WALK_BATCH candidates at a time (their position loads are in flight
@@ -855,7 +857,7 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // The cursor is branch-free: every slot reads the LDS range at the cursor's
 // (clamped) next index and selects it when the current range ends, and a
 // slot past the lane's last candidate names `self` (its load is a cached
-// re-read of the cell's own position; the filter skips it).
+// re-read of the cell's own position).
 #ifndef ONC_WALK_BATCH
 #define ONC_WALK_BATCH 4
 #endif
@@ -962,8 +964,7 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 tag, floa
 #pragma unroll
             for (int u = 0; u < ONC_WALK_BATCH; u++) p[u] = pos4[jj[u]];
 #pragma unroll
-            for (int u = 0; u < ONC_WALK_BATCH; u++)
-                if (jj[u] != self) pair(p[u], jj[u]);
+            for (int u = 0; u < ONC_WALK_BATCH; u++) pair(p[u], jj[u]);
             batch_end();
         }
     }
