@@ -831,7 +831,6 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 // and explicit overdamped integration. Reads no field, so it runs beside the
 // sweep.
 // ---------------------------------------------------------------------------
-#define ONC_PAIR_LIST 16
 #define ONC_FORCES_BLOCK 64  // one wave per block
 
 // Candidates of the 27 Moore bins around (px, py, pz), z-plane by z-plane, in
@@ -867,7 +866,16 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 #ifndef ONC_WALK_SELECT
 #define ONC_WALK_SELECT 1
 #endif
-static_assert(ONC_PAIR_LIST / 2 + ONC_WALK_BATCH <= ONC_PAIR_LIST, "a batch must fit the pair list");
+// A lane's pair list is flushed (wave-wide) once any lane holds more than
+// ONC_PAIR_FLUSH pairs after a batch, so it never holds more than
+// ONC_PAIR_FLUSH + ONC_WALK_BATCH.
+#ifndef ONC_PAIR_FLUSH
+#define ONC_PAIR_FLUSH 8
+#endif
+#define ONC_PAIR_LIST (ONC_PAIR_FLUSH + ONC_WALK_BATCH)
+#ifndef ONC_FORCES_WPE
+#define ONC_FORCES_WPE 5  // force-kernel waves per SIMD the register budget must allow
+#endif
 template <class F, class G>
 __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 tag, float px, float py, float pz,
                                             onc_u32 self, const float4* __restrict__ pos4, onc_u32 (*s_rs)[ONC_FORCES_BLOCK],
@@ -974,9 +982,11 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 tag, floa
 // [slot][thread] (bank-conflict free) and the non-empty bin ranges of the
 // current z-plane.
 // Range lengths are 16-bit (a bin of more than 65535 cells is reported by the
-// permute kernel as a device error): 7.5 KB per wave lets LDS hold as many
-// force waves as their registers allow (8.7 KB with 32-bit ends held the
-// kernel at 18 waves per CU instead of 20; forces 3.5 % faster).
+// permute kernel as a device error). With 12-entry pair lists a wave takes
+// 6.4 KB, so LDS holds as many force waves as their 78 VGPRs allow (24 per
+// CU). The walk needs about 21: padding LDS to 18 and 15 waves per CU made
+// the 1M-cell step 7 % and 11 % slower, while 24 ran as fast as 21, and 27
+// or 28 (72 VGPRs with a small spill, earlier flushes) 2-5 % slower.
 struct onc_forces_lds {
     onc_u32 pairs[ONC_PAIR_LIST][ONC_FORCES_BLOCK];
     onc_u32 rs[9][ONC_FORCES_BLOCK];
@@ -1041,7 +1051,7 @@ __device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u
     // Phase A: cheap distance filter over the 27 bins; only pairs inside
     // the interaction radius reach the expensive (sqrt, divide) path, so
     // lanes diverge on short lists instead of on every candidate. A list
-    // holds at most 8 entries between batches (a batch adds at most ONC_WALK_BATCH);
+    // holds at most ONC_PAIR_FLUSH entries between batches (a batch adds at most ONC_WALK_BATCH);
     // once any lane of the wave holds more, every lane still walking
     // flushes together. A flush taken by one lane at a time would run
     // the wave through the pair law once per lane.
@@ -1057,7 +1067,7 @@ __device__ __forceinline__ void onc_forces_cell(const onc_kargs& a, int s, onc_u
                     npair++;
                 },
                 [&]() {
-                    if (__ballot(npair > ONC_PAIR_LIST / 2) != 0ull) flush();
+                    if (__ballot(npair > ONC_PAIR_FLUSH) != 0ull) flush();
                 });
     flush();
     if (RAW) {
@@ -1096,14 +1106,14 @@ __device__ __forceinline__ void onc_forces_wave(const onc_kargs& a, onc_u32 blk,
 // entries, then candidate positions, then the listed neighbours): five waves
 // per SIMD instead of four (96 VGPRs, a 20-byte spill) run it 6% faster on the
 // dense 64-sim ensemble.
-__global__ __launch_bounds__(ONC_FORCES_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_forces(onc_kargs a) {
+__global__ __launch_bounds__(ONC_FORCES_BLOCK) __attribute__((amdgpu_waves_per_eu(ONC_FORCES_WPE, 8))) void k_forces(onc_kargs a) {
     __shared__ onc_forces_lds L;
     onc_forces_wave(a, blockIdx.x, gridDim.x, L);
 }
 
 // The summed forces alone (onc_forces): the same walk and pair law, written
 // out instead of integrated.
-__global__ __launch_bounds__(ONC_FORCES_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8)))
+__global__ __launch_bounds__(ONC_FORCES_BLOCK) __attribute__((amdgpu_waves_per_eu(ONC_FORCES_WPE, 8)))
 void k_forces_raw(onc_kargs a, float* fx, float* fy, float* fz) {
     __shared__ onc_forces_lds L;
     onc_for_chunks_xcd(a.cell_prefix, a.nsims, blockIdx.x, gridDim.x,
@@ -1489,7 +1499,7 @@ __global__ __launch_bounds__(256) void k_stencil(onc_kargs a) {
 // groups spread evenly among the force groups (a_.fs_sgroups of the
 // gridDim.x / 8 groups), so both parts keep their XCD-aware block orders and
 // proceed together through the dispatch order.
-__global__ __launch_bounds__(ONC_FORCES_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_forces_stencil(onc_kargs a) {
+__global__ __launch_bounds__(ONC_FORCES_BLOCK) __attribute__((amdgpu_waves_per_eu(ONC_FORCES_WPE, 8))) void k_forces_stencil(onc_kargs a) {
     __shared__ onc_forces_lds L;
     const onc_u32 G = gridDim.x >> 3, Gs = a.fs_sgroups;
     const onc_u32 pg = blockIdx.x >> 3, x = blockIdx.x & 7u;
