@@ -471,6 +471,11 @@ __device__ __forceinline__ void onc_for_chunks_xcd(const onc_u32* cell_prefix, o
     const onc_u32 wx = nblk >> 3;  // nblk is a multiple of 8 (>= 8): every XCD class has waves
     onc_u32 m = i;       // this wave's next chunk among XCD x's chunks
     onc_u32 before = 0;  // XCD x's chunks in the sims of earlier groups
+    // An XCD takes fewer than n / 512 + GG + 1 chunks of a sim of n cells
+    // (at most ceil(groups / 8) groups of GG chunks, each group but the
+    // last full): a wave past that bound for the whole batch has no chunk
+    // and leaves before the prefix scan.
+    if (i >= (cell_prefix[nsims] >> 9) + (GG + 1u) * nsims + 1u) return;
     for (onc_u32 g0 = 0; g0 < nsims; g0 += 64u) {
         // lane l: sim g0 + l, its cell count, and how many of its chunks XCD x takes
         const onc_u32 s = g0 + lane;
