@@ -18,6 +18,10 @@
 #define ONC_FLAG_P 0x80000000u
 #define ONC_CNT_MASK 0x3FFFFFFFu
 #define ONC_SPIN_LIMIT (1u << 24)
+// status words a sort tile's look-back reads per memory round trip
+#ifndef ONC_SORT_LOOKBACK
+#define ONC_SORT_LOOKBACK 8
+#endif
 #define ONC_F64_A (1ull << 62)
 #define ONC_F64_P (2ull << 62)
 #define ONC_M31 0x7FFFFFFFull
@@ -65,30 +69,40 @@ __device__ __forceinline__ onc_u32 block_exscan256(onc_u32 v, onc_u32* s_tmp, on
 }
 
 // Decoupled look-back for one digit of a sort tile: sums the aggregates of
-// tiles g-1, g-2, ... down to the nearest inclusive prefix, reading eight
-// status words per round so one memory round trip covers eight tiles. Tiles
-// below `first` (another sim) read as an inclusive prefix of 0.
+// tiles g-1, g-2, ... down to the nearest inclusive prefix, reading Q status
+// words per round so one memory round trip covers Q tiles. Tiles below
+// `first` (another sim) read as an inclusive prefix of 0. When every tile
+// publishes its aggregate at about the same time, the last of T tiles needs
+// about sqrt(2T/Q) rounds (340 tiles: 9 rounds at Q = 8, 5 at 32, 3 at 64).
+template <int Q>
 __device__ __forceinline__ onc_u32 lookback_digit(const onc_u32* status, onc_u32 g, onc_u32 first, onc_u32 d,
                                                   onc_u32* err) {
     onc_u32 prefix = 0u, j = g - 1u, spins = 0u;
     for (;;) {
-        onc_u32 v[8];
+        onc_u32 v[Q];
 #pragma unroll
-        for (int q = 0; q < 8; q++)
+        for (int q = 0; q < Q; q++)
             v[q] = (j >= first + (onc_u32)q) ? ld_relaxed(&status[(size_t)(j - (onc_u32)q) * 256u + d]) : ONC_FLAG_P;
+        // q = words taken before the first one not yet published (or through
+        // the first inclusive prefix); unrolled so v stays in registers
         int q = 0;
-        bool done = false;
-        for (; q < 8; q++) {
-            if ((v[q] & (ONC_FLAG_A | ONC_FLAG_P)) == 0u) break;
-            prefix += v[q] & ONC_CNT_MASK;
-            if (v[q] & ONC_FLAG_P) {
-                done = true;
-                break;
+        bool done = false, stop = false;
+#pragma unroll
+        for (int u = 0; u < Q; u++) {
+            if (!stop) {
+                const onc_u32 x = v[u];
+                if ((x & (ONC_FLAG_A | ONC_FLAG_P)) == 0u) {
+                    stop = true;
+                } else {
+                    prefix += x & ONC_CNT_MASK;
+                    q = u + 1;
+                    if (x & ONC_FLAG_P) done = stop = true;
+                }
             }
         }
         if (done) return prefix;
         j -= (onc_u32)q;
-        if (q < 8) {
+        if (q < Q) {
             if (++spins > ONC_SPIN_LIMIT) {
                 atomicOr(err, 1u);
                 return prefix;
@@ -515,33 +529,26 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
             if (valid && r == 0u) s_wc[w * 256 + d] = pre + (onc_u32)__popcll(m);
         }
         __syncthreads();
-        // per digit (thread = digit): wave prefix, tile count
+        // per digit (thread = digit): tile count, published at once so that
+        // later tiles can look past this one, then the wave prefixes
         const onc_u32 c0 = s_wc[tid], c1 = s_wc[256 + tid], c2 = s_wc[512 + tid], c3 = s_wc[768 + tid];
         const onc_u32 tot = c0 + c1 + c2 + c3;
+        onc_u32* mine = &status[(size_t)g * 256u + tid];
+        st_relaxed(mine, (t == 0u ? ONC_FLAG_P : ONC_FLAG_A) | tot);
         s_wc[tid] = 0u;
         s_wc[256 + tid] = c0;
         s_wc[512 + tid] = c0 + c1;
         s_wc[768 + tid] = c0 + c1 + c2;
         const onc_u32 texcl = block_exscan256(tot, s_tmp, nullptr);
         s_texcl[tid] = texcl;
-        // decoupled look-back over the sim's earlier tiles for digit tid
-        onc_u32 prefix = 0u;
-        onc_u32* mine = &status[(size_t)g * 256u + tid];
-        if (t == 0u) {
-            st_relaxed(mine, ONC_FLAG_P | tot);
-        } else {
-            st_relaxed(mine, ONC_FLAG_A | tot);
-            prefix = lookback_digit(status, g, g - t, (onc_u32)tid, a.err);
-            st_relaxed(mine, ONC_FLAG_P | (prefix + tot));
-        }
         onc_u32 hd = 0u;
 #pragma unroll
         for (int c = 0; c < ONC_HIST_COPIES; c++)
             hd += a.hist[(((size_t)c * a.nsims + (size_t)s) * 4u + (onc_u32)pass) * 256u + tid];
-        const onc_u32 gex = block_exscan256(hd, s_tmp, nullptr);
-        s_gbase[tid] = gex + prefix;
-        __syncthreads();
-        // reorder through LDS by digit
+        const onc_u32 gex = block_exscan256(hd, s_tmp, nullptr);  // ends with a barrier
+        // reorder through LDS by digit before the look-back: the positions
+        // inside the tile need no other tile, and the keys and values leave
+        // the registers that the look-back's window of loads then uses
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             onc_u32 idx = (onc_u32)(w * 64 * ITEMS + k * 64 + lane);
@@ -552,6 +559,13 @@ __global__ __launch_bounds__(256) void k_sort_pass(onc_kargs a, int pass) {
                 s_vals[pos] = val[k];
             }
         }
+        // decoupled look-back over the sim's earlier tiles for digit tid
+        onc_u32 prefix = 0u;
+        if (t != 0u) {
+            prefix = lookback_digit<ONC_SORT_LOOKBACK>(status, g, g - t, (onc_u32)tid, a.err);
+            st_relaxed(mine, ONC_FLAG_P | (prefix + tot));
+        }
+        s_gbase[tid] = gex + prefix;
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
