@@ -431,34 +431,14 @@ def serve(ctx, rate: float, seconds: float, budget_gb: float, progress_s: float 
     return out
 
 
-def _drift(a64, b32):
-    """Difference of binary32 results from binary64 ones after one step from
-    the same state: max absolute, and relative to max(|x64|, 1e-3): max, RMS,
-    99.9th percentile and the fraction of elements above 1e-5."""
-    import numpy as np
-    a = np.asarray(a64, dtype=np.float64)
-    b = np.asarray(b32, dtype=np.float64)
-    if a.size == 0:
-        return {"max_abs": 0.0, "max_rel": 0.0, "rms_rel": 0.0, "p999_rel": 0.0, "frac_rel_gt_1e-5": 0.0}
-    d = np.abs(a - b)
-    r = d / np.maximum(np.abs(a), 1e-3)
-    return {"max_abs": float(d.max()), "max_rel": float(r.max()), "rms_rel": float(np.sqrt(np.mean(r * r))),
-            "p999_rel": float(np.percentile(r, 99.9)), "frac_rel_gt_1e-5": float(np.mean(r > 1e-5))}
-
-
-def stress(ctx, steps: int = 300, burst=(100, 110), checkpoints=(0, 105)):
+def stress(ctx, steps: int = 300, burst=(100, 110)):
     """Stress workload: 16M cells in 32 Gaussian clusters, 512^3, three
     fields, 300 steps with a mitotic burst (division age 1 for steps
     [100, 110)). Timed as one captured graph per step (re-captured when the
-    parameters change). At each checkpoint k a fresh run is stepped to k,
-    downloaded, stepped once more and downloaded again; the oracle takes one
-    step from the same step-k state in binary32 (bit-exact parity at full
-    size) and in binary64 (per-step drift of positions, matched by cell id,
-    and of fields)."""
-    import numpy as np
+    parameters change). Its full-size parity and binary64 drift are checked
+    by tests/test_gpu_configs.py and reported by tests/drift_report.py."""
     from oncolytica_amd import workloads as W
     from oncolytica_amd.hip._sim import Batch, Sim
-    from oracle import oracle as O
     w = W.stress_c5()
     cols0 = W.initial_cells(w)
     base = list(w.params)
@@ -468,84 +448,31 @@ def stress(ctx, steps: int = 300, burst=(100, 110), checkpoints=(0, 105)):
     def params_at(k):
         return hot if burst[0] <= k < burst[1] else base
 
-    def run_to(k, timed=False):
-        sim = Sim(ctx, w.desc())
-        sim.upload_cells(cols0)
-        b = Batch(ctx)
-        b.add(sim)
+    sim = Sim(ctx, w.desc())
+    sim.upload_cells(cols0)
+    b = Batch(ctx)
+    b.add(sim)
+    b.sync()
+    done, ns = 0, []
+    t0 = time.perf_counter()
+    for edge in (burst[0], burst[1], steps):
+        if edge <= done:
+            continue
+        sim.set_params(params_at(done))
+        b.step(edge - done, graph=True)
+        done = edge
         b.sync()
-        done, ns = 0, []
-        t0 = time.perf_counter()
-        for edge in (burst[0], burst[1], k):
-            if edge <= done or done >= k:
-                continue
-            e = min(edge, k)
-            sim.set_params(params_at(done))
-            b.step(e - done, graph=True)
-            done = e
-            if timed:
-                b.sync()
-                ns.append((done, sim.counts()[0]))
-        b.sync()
-        return sim, b, time.perf_counter() - t0, ns
-
-    sim, b, wall, ns = run_to(steps, timed=True)
+        ns.append((done, sim.counts()[0]))
+    b.sync()
+    wall = time.perf_counter() - t0
     cu = _sim_cu(sim)
     n_final = sim.counts()[0]
     b.close()
     sim.close()
     out = {"workload": w.name, "cells_initial": w.n_cells, "grid": list(w.dims), "fields": len(w.fields),
            "steps": steps, "burst_steps": list(burst), "wall_s": wall, "cell_updates": cu,
-           "cell_updates_per_s": cu / wall, "live_cells_at": ns, "final_live_cells": n_final,
-           "checkpoints": []}
+           "cell_updates_per_s": cu / wall, "live_cells_at": ns, "final_live_cells": n_final}
     _log(f"stress: {cu / wall / 1e6:.0f} M cell-updates/s over {steps} steps ({wall:.2f} s), n at {ns}")
-    for k in checkpoints:
-        sim, b, _, _ = run_to(k)
-        ck = sim.download_cells()
-        fk = [sim.download_field(f) for f in range(len(w.fields))]
-        nid = sim.counts()[1]
-        sim.set_params(params_at(k))
-        b.step(1, graph=False)
-        b.sync()
-        g1 = sim.download_cells()
-        gf1 = [sim.download_field(f) for f in range(len(w.fields))]
-        b.close()
-        sim.close()
-        d = w.desc()
-        for i, p in enumerate(params_at(k)):
-            d.params[i] = p
-        t0 = time.perf_counter()
-        o32 = O.TumorState(d, ck, fields=fk, real="f32", next_id=nid)
-        o32.step.value = k
-        o32.run(1)
-        c32 = o32.columns()
-        same = len(c32[0]) == len(g1[0]) and all(
-            np.array_equal(a.view(np.uint32), g.view(np.uint32)) for a, g in zip(c32, g1)) and all(
-            np.array_equal(a.view(np.uint32), g.view(np.uint32)) for a, g in zip(o32.fields, gf1))
-        del o32
-        o64 = O.TumorState(d, ck, fields=fk, real="f64", next_id=nid)
-        o64.step.value = k
-        o64.run(1)
-        c64 = o64.columns()
-        # match the cells that existed at step k by id (a threshold decision
-        # can differ between precisions, and then the ids of this step's
-        # daughters, assigned by rank, no longer name the same cells)
-        ids32, ids64 = g1[3], c64[3]
-        common, i32, i64 = np.intersect1d(ids32, ids64, assume_unique=True, return_indices=True)
-        old_cells = common < nid
-        i32, i64 = i32[old_cells], i64[old_cells]
-        pos = _drift(np.concatenate([c64[c][i64] for c in range(3)]), np.concatenate([g1[c][i32] for c in range(3)]))
-        fld = [_drift(o64.fields[f], gf1[f]) for f in range(len(w.fields))]
-        del o64
-        ck_out = {"step": k, "cells": int(len(ck[0])), "bit_identical_to_oracle_f32": bool(same),
-                  "live_after_f32": int(len(ids32)), "live_after_f64": int(len(ids64)),
-                  "daughters_f32": int(np.sum(ids32 >= nid)), "daughters_f64": int(np.sum(ids64 >= nid)),
-                  "survivor_mismatches_vs_f64": int(np.sum(ids32 < nid) + np.sum(ids64 < nid) - 2 * len(i32)),
-                  "position_drift_vs_f64": pos, "field_drift_vs_f64": fld,
-                  "oracle_s": time.perf_counter() - t0}
-        out["checkpoints"].append(ck_out)
-        _log(f"stress: checkpoint {k}: {len(ck[0]):,} cells, bit-identical to oracle f32 {same}; "
-             f"f64 drift pos {pos}, fields {fld}")
     return out
 
 
@@ -777,8 +704,7 @@ def main():
     ap.add_argument("--stress-only", action="store_true",
                     help="run only the 16M-cell 512^3 stress workload and print its record as the JSON line")
     ap.add_argument("--stress", action="store_true",
-                    help="also run the 16M-cell 512^3 stress workload (300 steps, mitotic burst) with its "
-                         "full-size oracle parity and binary64 drift checkpoints")
+                    help="also run the 16M-cell 512^3 stress workload (300 steps, mitotic burst)")
     ap.add_argument("--long-steps", type=int, default=1000,
                     help="steps of the long run of the same workload (graph replays from its initial state, "
                          "through the hypoxic collapse to ~0.1M cells; 0: skip)")
