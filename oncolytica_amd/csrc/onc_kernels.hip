@@ -880,6 +880,10 @@ __global__ __launch_bounds__(256) void k_permute_bins(onc_kargs a) {
 #ifndef ONC_WALK_SELECT
 #define ONC_WALK_SELECT 1
 #endif
+#ifndef ONC_WALK_SENTINEL
+#define ONC_WALK_SENTINEL 1
+#endif
+#define ONC_WALK_SLOTS (9 + ONC_WALK_SENTINEL)  // range slots per lane: 9 bins (+ the sentinel)
 // A lane's pair list is flushed (wave-wide) once any lane holds more than
 // ONC_PAIR_FLUSH pairs after a batch, so it never holds more than
 // ONC_PAIR_FLUSH + ONC_WALK_BATCH.
@@ -949,16 +953,33 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 tag, floa
                 nr++;
             }
         }
+#if ONC_WALK_SENTINEL
+        // after the lane's last range, a range holding only the cell itself:
+        // the cursor stays there, so a spent slot names `self` with no test
+        s_rs[nr][tid] = self;
+        s_rl[nr][tid] = 1;
+        onc_u32 k = 0, j = s_rs[0][tid], e = j + s_rl[0][tid];
+#else
         onc_u32 k = 0, j = 0, e = 0;
         if (nr > 0) {
             j = s_rs[0][tid];
             e = j + s_rl[0][tid];
         }
+#endif
         while (k < nr) {
             onc_u32 jj[ONC_WALK_BATCH];
 #pragma unroll
             for (int u = 0; u < ONC_WALK_BATCH; u++) {
-#if ONC_WALK_SELECT
+#if ONC_WALK_SENTINEL
+                jj[u] = j;
+                const onc_u32 j1 = j + 1u;
+                const bool end = j1 == e;
+                k += end ? 1u : 0u;
+                const onc_u32 kk = k < nr ? k : nr;
+                const onc_u32 ns = s_rs[kk][tid], nl = s_rl[kk][tid];
+                j = end ? ns : j1;
+                e = end ? ns + nl : e;
+#elif ONC_WALK_SELECT
                 const bool v = k < nr;
                 jj[u] = v ? j : self;
                 const onc_u32 j1 = j + 1u;
@@ -994,17 +1015,17 @@ __device__ __forceinline__ void forces_walk(const onc_dsim& S, onc_u32 tag, floa
 
 // LDS of one forces wave: per-thread list of interacting neighbours
 // [slot][thread] (bank-conflict free) and the non-empty bin ranges of the
-// current z-plane.
+// current z-plane, followed by the sentinel range of the cell itself.
 // Range lengths are 16-bit (a bin of more than 65535 cells is reported by the
-// permute kernel as a device error). With 12-entry pair lists a wave takes
-// 6.4 KB, so LDS holds as many force waves as their 78 VGPRs allow (24 per
-// CU). The walk needs about 21: padding LDS to 18 and 15 waves per CU made
+// permute kernel as a device error). With 12-entry pair lists and ten range
+// slots a wave takes 6.75 KB, so LDS holds 23 force waves per CU (their 78
+// VGPRs would allow 24). The walk needs about 21: padding LDS to 18 and 15 waves per CU made
 // the 1M-cell step 7 % and 11 % slower, while 24 ran as fast as 21, and 27
 // or 28 (72 VGPRs with a small spill, earlier flushes) 2-5 % slower.
 struct onc_forces_lds {
     onc_u32 pairs[ONC_PAIR_LIST][ONC_FORCES_BLOCK];
-    onc_u32 rs[9][ONC_FORCES_BLOCK];
-    unsigned short rl[9][ONC_FORCES_BLOCK];  // range lengths
+    onc_u32 rs[ONC_WALK_SLOTS][ONC_FORCES_BLOCK];
+    unsigned short rl[ONC_WALK_SLOTS][ONC_FORCES_BLOCK];  // range lengths
 };
 
 // Forces of cell i of sim s through global memory (bins table, packed
