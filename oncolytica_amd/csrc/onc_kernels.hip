@@ -600,32 +600,43 @@ template <int ITEMS>
 __global__ __launch_bounds__(256) void k_sort_up(const onc_u32* __restrict__ keys, onc_u32 n, onc_u32 shift,
                                                  onc_u32* __restrict__ counts, onc_u32 T) {
     constexpr int TILE = 256 * ITEMS;
-    __shared__ onc_u32 s_h[256];
+    // one 256-bin LDS histogram per wave (fewer lanes on a bin's word); the
+    // tile's loads are all issued before the first count (3 % faster passes
+    // than a load and its four counts at a time)
+    constexpr int NH = 4;
+    __shared__ onc_u32 s_h[NH * 256];
     const int tid = threadIdx.x;
+    onc_u32* h = s_h + (tid >> 6) * 256;
     const onc_u32 t = onc_xcd_block(blockIdx.x, gridDim.x);  // each XCD a contiguous range of tiles
-    s_h[tid] = 0u;
+#pragma unroll
+    for (int c = 0; c < NH; c++) s_h[c * 256 + tid] = 0u;
     __syncthreads();
     const onc_u32 base = t * (onc_u32)TILE;
     if (base + (onc_u32)TILE <= n) {
         // whole tile: 16-byte loads (the key buffers are 256-byte aligned)
         typedef onc_u32 v4u __attribute__((ext_vector_type(4)));
+        v4u q[ITEMS / 4];
+#pragma unroll
+        for (int k = 0; k < ITEMS / 4; k++) q[k] = __builtin_nontemporal_load((const v4u*)(keys + base) + k * 256 + tid);
 #pragma unroll
         for (int k = 0; k < ITEMS / 4; k++) {
-            const v4u q = __builtin_nontemporal_load((const v4u*)(keys + base) + k * 256 + tid);
-            atomicAdd(&s_h[(q.x >> shift) & 255u], 1u);
-            atomicAdd(&s_h[(q.y >> shift) & 255u], 1u);
-            atomicAdd(&s_h[(q.z >> shift) & 255u], 1u);
-            atomicAdd(&s_h[(q.w >> shift) & 255u], 1u);
+            atomicAdd(&h[(q[k].x >> shift) & 255u], 1u);
+            atomicAdd(&h[(q[k].y >> shift) & 255u], 1u);
+            atomicAdd(&h[(q[k].z >> shift) & 255u], 1u);
+            atomicAdd(&h[(q[k].w >> shift) & 255u], 1u);
         }
     } else {
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             const onc_u32 i = base + (onc_u32)(k * 256 + tid);
-            if (i < n) atomicAdd(&s_h[(__builtin_nontemporal_load(keys + i) >> shift) & 255u], 1u);
+            if (i < n) atomicAdd(&h[(__builtin_nontemporal_load(keys + i) >> shift) & 255u], 1u);
         }
     }
     __syncthreads();
-    counts[(size_t)tid * T + t] = s_h[tid];
+    onc_u32 c = 0u;
+#pragma unroll
+    for (int w = 0; w < NH; w++) c += s_h[w * 256 + tid];
+    counts[(size_t)tid * T + t] = c;
 }
 
 template <int ITEMS>
