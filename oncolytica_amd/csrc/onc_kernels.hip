@@ -1575,19 +1575,42 @@ struct onc_tumor_out {
     onc_u32 id, gen, alive, div;
 };
 
-__device__ __forceinline__ onc_tumor_out onc_tumor_rule(const onc_dsim& S, onc_u32 i) {
+// The rule's inputs from the cell's columns: its integrated position
+// (current buffer) and id / age / generation (permuted buffer).
+struct onc_tumor_in {
+    float px, py, pz, age;
+    onc_u32 id, gen;
+};
+__device__ __forceinline__ onc_tumor_in onc_tumor_load(const onc_dsim& S, onc_u32 i) {
     const onc_u32 par = S.par;
     onc_gcol const* T = S.st[par ^ 1u];
     onc_gcol const* C = S.st[par];
+    onc_tumor_in c;
+    c.px = ((const float*)C[0])[i];
+    c.py = ((const float*)C[1])[i];
+    c.pz = ((const float*)C[2])[i];
+    c.id = T[3][i];
+    c.age = __uint_as_float(T[4][i]);
+    c.gen = T[5][i];
+    return c;
+}
+
+// Whether the rule reads the drug field (field 1) at the cell's voxel.
+__device__ __forceinline__ bool onc_tumor_reads_drug(const onc_dsim& S) {
+    return S.n_fields > 1 && S.params[3] > 0.0f;
+}
+
+// The rule given the cell's inputs and the swept O2 (field 0) and drug
+// (field 1; read only when onc_tumor_reads_drug) at its voxel.
+__device__ __forceinline__ onc_tumor_out onc_tumor_eval(const onc_dsim& S, const onc_tumor_in& c, float o2,
+                                                        float drug) {
     onc_tumor_out o;
-    o.px = ((const float*)C[0])[i];
-    o.py = ((const float*)C[1])[i];
-    o.pz = ((const float*)C[2])[i];
-    o.id = T[3][i];
-    float age = __uint_as_float(T[4][i]);
-    o.gen = T[5][i];
-    const onc_u32 v = onc_field_index(S, o.px, o.py, o.pz);
-    const float o2 = S.fld[par ^ 1u][0][v];
+    o.px = c.px;
+    o.py = c.py;
+    o.pz = c.pz;
+    o.id = c.id;
+    o.gen = c.gen;
+    float age = c.age;
     onc_rng rng;
     onc_rng_init(rng, S, o.id, S.step);
     age = age + 1.0f;
@@ -1599,8 +1622,7 @@ __device__ __forceinline__ onc_tumor_out onc_tumor_rule(const onc_dsim& S, onc_u
     if (o2 < S.params[2]) {
         o.alive = 0u;
     } else {
-        if (S.n_fields > 1 && S.params[3] > 0.0f) {
-            const float drug = S.fld[par ^ 1u][1][v];
+        if (onc_tumor_reads_drug(S)) {
             const float u = onc_rand(rng);
             if (u < S.params[3] * drug) o.alive = 0u;
         }
@@ -1618,6 +1640,14 @@ __device__ __forceinline__ onc_tumor_out onc_tumor_rule(const onc_dsim& S, onc_u
     }
     o.age = age;
     return o;
+}
+
+__device__ __forceinline__ onc_tumor_out onc_tumor_rule(const onc_dsim& S, onc_u32 i) {
+    const onc_tumor_in c = onc_tumor_load(S, i);
+    const onc_u32 v = onc_field_index(S, c.px, c.py, c.pz);
+    const float o2 = S.fld[S.par ^ 1u][0][v];
+    const float drug = onc_tumor_reads_drug(S) ? S.fld[S.par ^ 1u][1][v] : 0.0f;
+    return onc_tumor_eval(S, c, o2, drug);
 }
 
 __global__ __launch_bounds__(256) void k_rule_tumor(onc_rule_args r) {
@@ -1812,14 +1842,48 @@ __global__ __launch_bounds__(256) void k_rule_compact(onc_kargs a) {
         const onc_u32 base = t * (onc_u32)TILE;
         const onc_u32 cnt = min((onc_u32)TILE, n - base);
         const onc_u32 par = S.par;
+        // the rule for the tile's cells. Tiles of up to four cells per thread
+        // (small populations, where the kernel's latency is its cost) take
+        // all their items' loads in two rounds (the columns, then the fields
+        // at the cells' voxels) instead of two dependent rounds per item: the
+        // 1,000-step run of the 1M-cell workload 1.5 % faster. Larger tiles
+        // go item by item: holding eight items' inputs took 127 -> 138-164
+        // VGPRs (four waves per SIMD -> three) and the kernel from 35 to
+        // 37-38 us at 1.4M cells.
+        constexpr int PG = ITEMS <= 4 ? ITEMS : 1;
+        const bool rd = onc_tumor_reads_drug(S);
+        const float* __restrict__ f0 = S.fld[par ^ 1u][0];
+        const float* __restrict__ f1 = rd ? S.fld[par ^ 1u][1] : f0;
         onc_tumor_out o[ITEMS];
 #pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            const onc_u32 j = (onc_u32)(k * 256 + tid);
-            o[k].alive = 0u;
-            o[k].div = 0u;
-            if (j < cnt) o[k] = onc_tumor_rule(S, base + j);
-            s_aslot[j] = o[k].alive | (o[k].div << 1);
+        for (int k0 = 0; k0 < ITEMS; k0 += PG) {
+            onc_tumor_in cin[PG];
+#pragma unroll
+            for (int q = 0; q < PG; q++) {
+                const onc_u32 j = (onc_u32)((k0 + q) * 256 + tid);
+                if (j < cnt) cin[q] = onc_tumor_load(S, base + j);
+            }
+            float o2[PG], drug[PG];
+#pragma unroll
+            for (int q = 0; q < PG; q++) {
+                const onc_u32 j = (onc_u32)((k0 + q) * 256 + tid);
+                o2[q] = 0.0f;
+                drug[q] = 0.0f;
+                if (j < cnt) {
+                    const onc_u32 v = onc_field_index(S, cin[q].px, cin[q].py, cin[q].pz);
+                    o2[q] = f0[v];
+                    if (rd) drug[q] = f1[v];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < PG; q++) {
+                const int k = k0 + q;
+                const onc_u32 j = (onc_u32)(k * 256 + tid);
+                o[k].alive = 0u;
+                o[k].div = 0u;
+                if (j < cnt) o[k] = onc_tumor_eval(S, cin[q], o2[q], drug[q]);
+                s_aslot[j] = o[k].alive | (o[k].div << 1);
+            }
         }
         compact_slots<ITEMS>(a, g, S, t, ntiles_s, s_aslot, s_dslot, s_tmp, s_pa, s_pd);
         onc_gcol const* D = S.st[par ^ 1u];
