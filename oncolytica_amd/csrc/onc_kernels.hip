@@ -1842,15 +1842,16 @@ __global__ __launch_bounds__(256) void k_rule_compact(onc_kargs a) {
         const onc_u32 base = t * (onc_u32)TILE;
         const onc_u32 cnt = min((onc_u32)TILE, n - base);
         const onc_u32 par = S.par;
-        // the rule for the tile's cells. Tiles of up to four cells per thread
-        // (small populations, where the kernel's latency is its cost) take
-        // all their items' loads in two rounds (the columns, then the fields
-        // at the cells' voxels) instead of two dependent rounds per item: the
-        // 1,000-step run of the 1M-cell workload 1.5 % faster. Larger tiles
-        // go item by item: holding eight items' inputs took 127 -> 138-164
-        // VGPRs (four waves per SIMD -> three) and the kernel from 35 to
-        // 37-38 us at 1.4M cells.
-        constexpr int PG = ITEMS <= 4 ? ITEMS : 1;
+        // the rule for the tile's cells, PG items at a time with their loads
+        // in two rounds (the columns, then the fields at the cells' voxels)
+        // instead of two dependent rounds per item. Tiles of up to four cells
+        // per thread (small populations, where the kernel's latency is its
+        // cost) take all their items at once: the 1,000-step run of the
+        // 1M-cell workload 1.5 % faster. Larger tiles take pairs: eight items
+        // (or four) at once took 127 -> 164 (138) VGPRs, three waves per SIMD
+        // instead of four, and the kernel from 35 to 37-38 us at 1.4M cells;
+        // pairs keep 126 VGPRs and took it to 34.3 us.
+        constexpr int PG = ITEMS <= 4 ? ITEMS : 2;
         const bool rd = onc_tumor_reads_drug(S);
         const float* __restrict__ f0 = S.fld[par ^ 1u][0];
         const float* __restrict__ f1 = rd ? S.fld[par ^ 1u][1] : f0;
